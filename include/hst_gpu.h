@@ -1,0 +1,163 @@
+/*
+ * hst_gpu.h -- C ABI of libhstgpu.so, the MI355X (gfx950) implementation of
+ * the RK3 substep of hst (homogeneous shear turbulence DNS).
+ *
+ * The Fortran host binds these with iso_c_binding interface blocks (module
+ * hst_gpu_api); Python binds them with ctypes.  Every entry point returns 0
+ * on success and a non-zero status on failure; the text of the last failure
+ * is available from hst_gpu_last_error().  The reference's convention (every
+ * error is fatal, `error stop`) is kept by the Fortran wrappers, which stop
+ * on a non-zero status.
+ *
+ * Device state is process-global (one GPU per process, like one device per
+ * rank in the reference): hst_gpu_init allocates it, hst_gpu_free releases
+ * it.  Nothing is re-entrant; all work is in order on one HIP stream.
+ *
+ * Field identifiers name the device-resident arrays of the reference's
+ * module hst_params (src/hst_params.f90:101-103):
+ *   HST_V1..HST_V3      V(:, :, :, 1:3)      u, v, w
+ *   HST_RHS1, HST_RHS2  rhs(:, :, :, 1:2)    eta and d2v right-hand sides
+ *   HST_OLD1, HST_OLD2  oldrhs(:, :, :, 1:2) carried explicit terms
+ *   HST_SCRATCH         a spare field of the same shape (tests, pressure)
+ * Host copies of a field use the reference layout field(iy, iz, ix),
+ * iy = -2..ny+1 fastest, then iz = -nz..nz, then ix = 0..nx (complex128).
+ */
+#ifndef HST_GPU_H
+#define HST_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The deck values the substep needs: the namelist groups &mesh, &physics,
+ * &init of hst.in (src/hst_input.f90:19-23, defaults :26-42).  Logicals are
+ * 0/1.  The hooks that are off in every deck this library serves (Stokes
+ * layer, large-scale damping) must be zero; hst_gpu_init rejects them. */
+typedef struct hst_deck {
+  int32_t nx, ny, nz;              /* &mesh: modes in x, points in y, modes in z */
+  int32_t linear, exact_shift;     /* &physics logicals */
+  int32_t seed, tg_mode;           /* &init */
+  int32_t initial_condition;       /* 0: 'random', 1: 'taylor_green' */
+  double alfa0, beta0, ly, ystretch;
+  double re, S, s2_mean, s2_amplitude, s2_period, s2_start;
+  double amplitude, kpeak, tg_amplitude;
+  double sl_amplitude;             /* must be 0 (Stokes layer, src/hst_stokes.f90) */
+  double damp_kpar, damp_kperp;    /* must be 0 (damping, src/hst_damping.f90) */
+} hst_deck;
+
+/* Derived sizes and grid tables (src/hst_input.f90:92-115, src/hst_setup.f90:42-64). */
+typedef struct hst_grid_info {
+  int32_t nxd, nzd;                /* 3/2-padded sizes: 2*nxd real x points, nzd z points */
+  int32_t nxm, nzm, nyg;           /* nx+1 x modes, 2nz+1 z modes, ny+4 rows with ghosts */
+  int32_t ncol;                    /* nxm*nzm Fourier columns */
+  double lx, lz, dx, dy, dz, factor, ni;
+} hst_grid_info;
+
+enum {
+  HST_V1 = 1, HST_V2 = 2, HST_V3 = 3,
+  HST_RHS1 = 4, HST_RHS2 = 5, HST_OLD1 = 6, HST_OLD2 = 7,
+  HST_SCRATCH = 8
+};
+
+/* System kinds of line_solve (src/hst_linsolve.f90:83). */
+enum { HST_KIND_D2V = 1, HST_KIND_ETA = 2, HST_KIND_POISSON = 3, HST_KIND_D0 = 4, HST_KIND_DY = 5 };
+
+/* Arithmetic of the transforms, products and derivative right-hand sides. */
+enum { HST_FP64 = 0, HST_MIXED = 1 };
+
+/* ------------------------------------------------------------------ host-only (no GPU) */
+
+/* Derived parameters, y grid, dyl and the compact weights der(0:ny-1, 0:3, -2:2)
+ * (Fortran order, iy fastest): read_input's derived block (src/hst_input.f90:92-115),
+ * allocate_fields' grid (src/hst_setup.f90:42-53), setup_derivatives
+ * (src/hst_derivatives.f90:30-56).  y has ny+4 entries (iy = -2..ny+1), dyl ny,
+ * der 20*ny.  Any output pointer may be NULL. */
+int hst_grid_setup(const hst_deck *deck, hst_grid_info *info, double *y, double *dyl, double *der);
+
+/* The seeded random field or the Taylor-Green field into a host array in the
+ * reference layout, ghost rows included: generate_initial_field
+ * (src/hst_initial.f90:27-67, 76-108). */
+int hst_initial_field(const hst_deck *deck, double *V_host /* complex128[3][nx+1][2nz+1][ny+4] */);
+
+/* ------------------------------------------------------------------ device state */
+
+/* allocate_fields + init_fft + init_linsolve + setup_derivatives
+ * (src/hst_setup.f90:42, src/hst_fft.f90:75, src/hst_linsolve.f90:99,
+ *  src/hst_derivatives.f90:30).  precision: HST_FP64 or HST_MIXED. */
+int hst_gpu_init(const hst_deck *deck, int precision);
+/* free_fields + free_fft + free_linsolve (src/hst_setup.f90:73, hst_fft.f90:317). */
+int hst_gpu_free(void);
+/* device_sync (src/hst_fft.f90:343). */
+int hst_gpu_sync(void);
+/* Text of the last failure (NUL-terminated, truncated to len). */
+int hst_gpu_last_error(char *buf, int len);
+/* Grid tables as computed at init (same as hst_grid_setup). */
+int hst_gpu_grid(hst_grid_info *info, double *y, double *dyl, double *der);
+
+/* `!$omp target update to/from` of one field (src/hst.f90:237,271; hst_io.f90:173). */
+int hst_gpu_upload_field(int field, const double *host);
+int hst_gpu_download_field(int field, double *host);
+/* field = 0 (zero-fill) helper for rhs/oldrhs at set-up (src/hst_setup.f90:67-69). */
+int hst_gpu_zero_field(int field);
+
+/* ------------------------------------------------------------------ the substep's pieces */
+
+/* fill_ghosts / fill_ghosts_field -> exchange_ghost_rows -> wrap_rows with
+ * npy = 1 (src/hst_derivatives.f90:134-157, src/hst_mpi.f90:273,340-355). */
+int hst_gpu_fill_ghosts(int field, double shift_x, double shift_z);
+
+/* transform_to_physical (src/hst_transforms.f90:74-90), then the six products
+ * (build_products, :97-121, both groups) and products_to_spectral (:125-139):
+ * the product spectra of both groups are left on the device.  When cfl is
+ * non-NULL the CFL number of the physical velocity is computed on the way
+ * (compute_cfl, :163-179) and returned. */
+int hst_gpu_nonlinear(double *cfl);
+/* transform_to_physical + compute_cfl only (the linear=.true. substep 3). */
+int hst_gpu_compute_cfl(double *cfl);
+/* vvdz_to_field (src/hst_transforms.f90:142-157): product p (1..6, in the order
+ * uu vv ww uv vw uw) of the last hst_gpu_nonlinear into a field. */
+int hst_gpu_product_to_field(int p, int field);
+/* The round trip of test_roundtrip (tests/test_roundtrip.f90:52-77): V -> physical
+ * -> (times factor) -> spectral, left as products 1..3. */
+int hst_gpu_roundtrip(void);
+
+/* buildrhs_prepare (src/hst_equations.f90:92-141) at `time` with step deltat. */
+int hst_gpu_buildrhs_prepare(const double ode[3], double deltat, double time);
+/* buildrhs(ODE, g) (src/hst_equations.f90:151-213) from the product spectra of group g. */
+int hst_gpu_buildrhs(const double ode[3], int g);
+/* shear_shift(dt_sub) at `time` (src/hst_equations.f90:226-293): default node
+ * phase or, with exact_shift, the D0-unweighted shift. */
+int hst_gpu_shear_shift(double dt_sub, double time);
+/* line_solve(sys, lambda, src, dst, shift_x, shift_z) (src/hst_linsolve.f90:147-233). */
+int hst_gpu_line_solve(int kind, double lambda, int src, int dst, double shift_x, double shift_z);
+/* linsolve(lambda) at `time` (src/hst_equations.f90:298-326): solves, ghosts, u and w. */
+int hst_gpu_linsolve(double lambda, double time);
+
+/* timestep() (src/hst_equations.f90:56-87): one RK3 step of length deltat.
+ * *time is advanced in FP64 by 2/RK_rai(1,i)*deltat per substep, as the
+ * reference does; *cfl receives the substep-3 CFL number (may be NULL). */
+int hst_gpu_timestep(double deltat, double *time, double *cfl);
+
+/* outstats' sums (src/hst_stats.f90:136-246), npy = 1, no Stokes layer or
+ * damping: out[0..8] = energy, diss, uv, vw, uu, vv, ww, uw (box means,
+ * as written to Runtimedata / variances_runtime.dat) and mfx, mfz. */
+int hst_gpu_outstats(double out[10]);
+
+/* ------------------------------------------------------------------ measurement */
+
+/* Per-kernel timing with HIP events on the library's stream (hst_timer's
+ * phases, src/hst_timer.f90, at kernel granularity).  enable = 1 starts
+ * recording (and clears the table), 0 stops. */
+int hst_gpu_profile(int enable);
+/* Number of kernels in the table; name, launches, total milliseconds and the
+ * algorithmic HBM bytes of all launches of entry i. */
+int hst_gpu_profile_count(void);
+int hst_gpu_profile_entry(int i, char *name, int len, int64_t *launches, double *ms, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HST_GPU_H */
