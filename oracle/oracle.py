@@ -179,3 +179,11 @@ def run_deck(deck: Deck, nsteps: int | None = None) -> Oracle:
         istep += 1
     o.nsteps = istep
     return o
+
+
+def _reset_old(self) -> None:
+    self.field(6)[...] = 0.0
+    self.field(7)[...] = 0.0
+
+
+Oracle.reset_old = _reset_old
