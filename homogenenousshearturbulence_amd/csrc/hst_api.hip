@@ -1,0 +1,642 @@
+// The C ABI of libhstgpu (include/hst_gpu.h): device state, the entry points
+// of the RK3 substep and the per-kernel event timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hst_cx.h"
+#include "hst_internal.h"
+
+namespace hst {
+
+static std::string g_error;
+static std::mutex g_error_mutex;
+void set_error(const std::string &msg) {
+  std::lock_guard<std::mutex> lk(g_error_mutex);
+  g_error = msg;
+}
+
+// RK3 (Rai-Moin) coefficients, one row per substep: (unknown/deltat, new
+// explicit, old explicit); substep lengths 2/RK(i,0) deltat = 8/15, 2/15, 1/3.
+static const double RK[3][3] = {{120.0 / 32.0, 2.0, 0.0},
+                                {120.0 / 8.0, 50.0 / 8.0, 34.0 / 8.0},
+                                {120.0 / 20.0, 90.0 / 20.0, 50.0 / 20.0}};
+static const double TWOPI = 6.283185307179586;
+
+// ------------------------------------------------------------------ profiling
+
+struct ProfEntry {
+  int64_t launches = 0;
+  double ms = 0.0, bytes = 0.0;
+};
+struct Pending {
+  std::string name;
+  hipEvent_t a, b;
+  double bytes;
+};
+
+struct State {
+  bool live = false;
+  hst_deck deck;
+  int mixed = 0;
+  hst_grid_info g;
+  Dims D;
+  std::vector<double> y, dyl, der;
+  D0Factor f0;
+  D0Const fc;
+  double *d_der = nullptr, *d_y = nullptr, *d_dyl = nullptr, *d_f0 = nullptr, *d_part = nullptr;
+  size_t part_cap = 0;
+  double2 *F[9] = {nullptr};
+  void *P = nullptr, *Hv = nullptr, *Hp = nullptr, *ws = nullptr;
+  size_t ws_bytes = 0;
+  int chunk = 0;
+  PentaWS pws[2];
+  double2 *XR[4];
+  void *tables = nullptr;
+  hipStream_t stream = nullptr;
+  int64_t bytes_held = 0;
+  bool prof = false;
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, ProfEntry> table;
+  std::vector<std::string> order;
+};
+
+static State S;
+static int g_chunk_request = 0;
+
+static int fail(const std::string &what, int rc = 1) {
+  set_error(what);
+  return rc ? rc : 1;
+}
+
+static int hip_fail(const char *where, hipError_t e) {
+  return fail(std::string(where) + ": " + hipGetErrorString(e), (int)e);
+}
+
+#define HIPCHK(call)                                          \
+  do {                                                        \
+    hipError_t _e = (call);                                   \
+    if (_e != hipSuccess) return hip_fail(#call, _e);         \
+  } while (0)
+
+static hipEvent_t take_event() {
+  if (!S.pool.empty()) {
+    hipEvent_t e = S.pool.back();
+    S.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+static void flush_profile() {
+  if (S.pending.empty()) return;
+  (void)hipStreamSynchronize(S.stream);
+  for (auto &p : S.pending) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, p.a, p.b);
+    if (!S.table.count(p.name)) S.order.push_back(p.name);
+    ProfEntry &e = S.table[p.name];
+    e.launches++;
+    e.ms += ms;
+    e.bytes += p.bytes;
+    S.pool.push_back(p.a);
+    S.pool.push_back(p.b);
+  }
+  S.pending.clear();
+}
+
+// Runs one launcher; with timing on, brackets it with events on the stream.
+template <class F>
+static int run(const char *name, double bytes, F launch) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (S.prof) {
+    a = take_event();
+    b = take_event();
+    (void)hipEventRecord(a, S.stream);
+  }
+  int rc = launch();
+  if (rc) return fail(std::string("launch of ") + name + " failed: " + hipGetErrorString((hipError_t)rc), rc);
+  if (S.prof) {
+    (void)hipEventRecord(b, S.stream);
+    S.pending.push_back(Pending{name, a, b, bytes});
+    if (S.pending.size() > 4096) flush_profile();
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ clock
+
+static double s2_of(double t) {
+  const hst_deck &d = S.deck;
+  if ((d.s2_amplitude == 0.0 && d.s2_mean == 0.0) || t < d.s2_start) return 0.0;
+  if (d.s2_period > 0.0) return d.s2_mean + d.s2_amplitude * std::sin(TWOPI / d.s2_period * (t - d.s2_start));
+  return d.s2_mean + d.s2_amplitude;
+}
+
+static double s2_integral(double t1, double t2) {
+  const hst_deck &d = S.deck;
+  if ((d.s2_amplitude == 0.0 && d.s2_mean == 0.0) || t2 <= d.s2_start) return 0.0;
+  const double a = std::max(t1, d.s2_start), b = t2;
+  if (d.s2_period > 0.0) {
+    const double omega = TWOPI / d.s2_period;
+    return d.s2_mean * (b - a) +
+           d.s2_amplitude / omega * (std::cos(omega * (a - d.s2_start)) - std::cos(omega * (b - d.s2_start)));
+  }
+  return (d.s2_mean + d.s2_amplitude) * (b - a);
+}
+
+// Fortran MODULO of reals (fmod, then the sign fix)
+static double modulo_r(double a, double p) {
+  double r = std::fmod(a, p);
+  if (r != 0.0 && ((r < 0.0) != (p < 0.0))) r += p;
+  return r;
+}
+
+static void shear_shifts(double t, double &sx, double &sz) {
+  sx = modulo_r(S.deck.S * t * S.deck.ly, S.g.lx);
+  sz = modulo_r(s2_integral(S.deck.s2_start, t) * S.deck.ly, S.g.lz);
+}
+
+// ------------------------------------------------------------------ pieces
+
+static const void *P_of(int p) {  // product p = 0..5
+  const size_t cs = S.mixed ? sizeof(float2) : sizeof(double2);
+  return (const char *)S.P + (size_t)p * S.D.NYg * S.D.ncol * cs;
+}
+
+static int nonlinear_pass(int mode, bool want_cfl, double *cfl) {
+  const double2 *V[3] = {S.F[1], S.F[2], S.F[3]};
+  const int np = (mode == 0 ? 6 : 3);
+  int npart_total = 0;
+  for (int row0 = 0; row0 < S.D.NYg; row0 += S.chunk) {
+    const int nrows = std::min(S.chunk, S.D.NYg - row0);
+    int rc = run("zift", bytes_zift(S.D, S.mixed, nrows),
+                 [&] { return launch_zift(S.D, S.mixed, V, S.Hv, row0, nrows, S.tables, S.stream); });
+    if (rc) return rc;
+    int npart = 0;
+    double *part = S.d_part + npart_total;
+    rc = run(mode == 2 ? "xpass_cfl" : "xpass", bytes_xpass(S.D, S.mixed, mode, nrows), [&] {
+      return launch_xpass(S.D, S.mixed, mode, S.Hv, S.Hp, row0, nrows, S.tables, S.d_dyl, part, want_cfl ? 1 : 0,
+                          &npart, S.stream);
+    });
+    if (rc) return rc;
+    if (want_cfl) npart_total += npart;
+    if (mode == 2) continue;
+    rc = run("zfft", bytes_zfft(S.D, S.mixed, np, nrows), [&] {
+      return launch_zfft(S.D, S.mixed, np, S.Hp, S.P, (size_t)S.D.NYg * S.D.ncol, row0, nrows, S.tables, S.stream);
+    });
+    if (rc) return rc;
+  }
+  if (want_cfl && cfl) {
+    std::vector<double> h(npart_total);
+    HIPCHK(hipMemcpyAsync(h.data(), S.d_part, sizeof(double) * npart_total, hipMemcpyDeviceToHost, S.stream));
+    HIPCHK(hipStreamSynchronize(S.stream));
+    double c = 0.0;
+    for (double v : h) c = std::max(c, v);
+    *cfl = c;
+  }
+  return 0;
+}
+
+static double bytes_field() { return (double)S.D.ny * S.D.ncol * 16.0; }
+
+static int buildrhs(const double ode[3], double deltat, double time, int mode, bool shift, double dgx, double dgz) {
+  RhsArgs a;
+  a.ode0 = ode[0]; a.ode1 = ode[1]; a.ode2 = ode[2];
+  a.deltat = deltat;
+  a.s2now = s2_of(time);
+  a.dgx = dgx; a.dgz = dgz;
+  a.nonlinear = mode;
+  a.shift = shift ? 1 : 0;
+  const void *P[6];
+  for (int p = 0; p < 6; p++) P[p] = P_of(p);
+  // prepare: u, v, w and oldrhs in, rhs and oldrhs out (9 fields); a group adds
+  // its three product spectra and reads and writes rhs, oldrhs (8 fields)
+  const double ps = S.mixed ? 8.0 : 16.0, pf = (double)S.D.ny * S.D.ncol * ps;
+  double bytes = bytes_field() * 9.0;
+  if (mode == 3) bytes += 6.0 * pf;
+  else if (mode != 0) bytes = bytes_field() * 8.0 + 3.0 * pf;
+  return run(mode == 3 ? "buildrhs" : (mode == 0 ? "buildrhs_prepare" : "buildrhs_group"), bytes, [&] {
+    return launch_buildrhs(S.D, S.mixed, a, S.F[1], S.F[2], S.F[3], P, S.F[HST_RHS1], S.F[HST_RHS2], S.F[HST_OLD1],
+                           S.F[HST_OLD2], S.d_der, S.d_y, S.stream);
+  });
+}
+
+static int exact_shift(double dt_sub, double time) {
+  const double dgx = S.deck.S * dt_sub, dgz = s2_integral(time, time + dt_sub);
+  double sx0, sz0, sx1, sz1;
+  shear_shifts(time, sx0, sz0);
+  shear_shifts(time + dt_sub, sx1, sz1);
+  double2 *q[4] = {S.F[HST_RHS1], S.F[HST_RHS2], S.F[HST_OLD1], S.F[HST_OLD2]};
+  return run("exact_shift", 4.0 * 4.0 * bytes_field(), [&] {
+    return launch_exact_shift(S.D, 4, q, S.XR, S.d_f0, S.fc, S.d_der, S.d_y, dgx, dgz, sx0, sz0, sx1, sz1, S.stream);
+  });
+}
+
+static int shear_shift(double dt_sub, double time) {
+  if (S.deck.exact_shift) return exact_shift(dt_sub, time);
+  const double dgx = S.deck.S * dt_sub, dgz = s2_integral(time, time + dt_sub);
+  return run("phase_shift", 8.0 * bytes_field(), [&] {
+    return launch_phase4(S.D, S.F[HST_RHS1], S.F[HST_RHS2], S.F[HST_OLD1], S.F[HST_OLD2], S.d_y, dgx, dgz, S.stream);
+  });
+}
+
+static int ghosts(int nf, double2 *const *f, double sx, double sz) {
+  return run("ghosts", nf * 8.0 * S.D.ncol * 16.0, [&] { return launch_ghosts(S.D, nf, f, sx, sz, S.stream); });
+}
+
+static int linsolve(double lambda, double time) {
+  double sx, sz;
+  shear_shifts(time, sx, sz);
+  const int kinds[2] = {HST_KIND_D2V, HST_KIND_ETA};
+  const double2 *src[2] = {S.F[HST_RHS2], S.F[HST_RHS1]};
+  double2 *dst[2] = {S.F[HST_V2], S.F[HST_V1]};
+  int rc = run("penta_d2v_eta", 2.0 * 8.0 * bytes_field(), [&] {
+    return launch_penta(S.D, 2, kinds, lambda, src, dst, S.pws, S.d_der, sx, sz, S.stream);
+  });
+  if (rc) return rc;
+  double2 *g2[1] = {S.F[HST_V2]};
+  if ((rc = ghosts(1, g2, sx, sz))) return rc;
+  rc = run("dy_recover", 6.0 * bytes_field(), [&] {
+    return launch_d0solve(S.D, 1, 1, S.F[HST_V2], S.F[HST_V1], S.F[HST_V3], S.XR[0], S.d_f0, S.fc, S.d_der, S.d_dyl,
+                          sx, sz, nullptr, nullptr, S.stream);
+  });
+  if (rc) return rc;
+  double2 *g13[2] = {S.F[HST_V1], S.F[HST_V3]};
+  return ghosts(2, g13, sx, sz);
+}
+
+static int check_live() {
+  if (!S.live) return fail("hst_gpu_init has not been called");
+  return 0;
+}
+
+static int check_field(int f) {
+  if (f < 1 || f > 8) return fail("field id out of range (1..8)");
+  return 0;
+}
+
+static void release() {
+  if (S.stream) (void)hipStreamSynchronize(S.stream);
+  for (int i = 1; i <= 8; i++) if (S.F[i]) (void)hipFree(S.F[i]);
+  void *ptrs[] = {S.P, S.Hv, S.Hp, S.ws, S.d_der, S.d_y, S.d_dyl, S.d_f0, S.d_part};
+  for (void *p : ptrs) if (p) (void)hipFree(p);
+  if (S.tables) free_fft_tables(S.tables);
+  for (auto &p : S.pending) { S.pool.push_back(p.a); S.pool.push_back(p.b); }
+  for (auto e : S.pool) (void)hipEventDestroy(e);
+  if (S.stream) (void)hipStreamDestroy(S.stream);
+  S = State();
+}
+
+}  // namespace hst
+
+using namespace hst;
+
+extern "C" {
+
+int hst_gpu_last_error(char *buf, int len) {
+  std::lock_guard<std::mutex> lk(g_error_mutex);
+  if (!buf || len <= 0) return 1;
+  std::snprintf(buf, (size_t)len, "%s", g_error.c_str());
+  return 0;
+}
+
+int hst_gpu_set_chunk_rows(int rows) {
+  if (rows < 0) return fail("chunk rows must be >= 0");
+  g_chunk_request = rows;
+  return 0;
+}
+
+int hst_gpu_memory(int64_t *bytes, int *chunk_rows) {
+  if (bytes) *bytes = S.bytes_held;
+  if (chunk_rows) *chunk_rows = S.chunk;
+  return 0;
+}
+
+int hst_gpu_init(const hst_deck *deck, int precision) {
+  if (!deck) return fail("deck is NULL");
+  if (S.live) release();
+  std::string err = check_deck(*deck);
+  if (!err.empty()) return fail(err);
+  if (precision != HST_FP64 && precision != HST_MIXED) return fail("precision must be HST_FP64 or HST_MIXED");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device");
+  S.deck = *deck;
+  S.mixed = precision == HST_MIXED;
+  grid_setup(S.deck, S.g, S.y, S.dyl, S.der);
+  if (!fft_size_supported(S.g.nzd)) return fail("unsupported z transform size nzd = " + std::to_string(S.g.nzd));
+  if (!fft_size_supported(S.g.nxd)) return fail("unsupported x transform size nxd = " + std::to_string(S.g.nxd));
+  Dims &D = S.D;
+  D.nx = deck->nx; D.ny = deck->ny; D.nz = deck->nz;
+  D.NXm = S.g.nxm; D.NZm = S.g.nzm; D.NYg = S.g.nyg; D.ncol = S.g.ncol;
+  D.nxd = S.g.nxd; D.nzd = S.g.nzd;
+  D.NXp = (D.NXm + 15) / 16 * 16;
+  D.alfa0 = deck->alfa0; D.beta0 = deck->beta0; D.ni = S.g.ni; D.S = deck->S;
+  D.factor = S.g.factor; D.dx = S.g.dx; D.dz = S.g.dz;
+  d0_factor(S.der, D.ny, S.f0);
+  std::memcpy(S.fc.rec, S.f0.rec, sizeof(S.fc.rec));
+  S.fc.c1m4 = S.f0.c1m4; S.fc.c1m3 = S.f0.c1m3; S.fc.d11 = S.f0.d11; S.fc.d12 = S.f0.d12; S.fc.c10 = S.f0.c10;
+  S.fc.c2m3 = S.f0.c2m3; S.fc.d21 = S.f0.d21; S.fc.d22 = S.f0.d22; S.fc.c20 = S.f0.c20; S.fc.c21 = S.f0.c21;
+
+  HIPCHK(hipStreamCreate(&S.stream));
+  const size_t nf = (size_t)D.NYg * D.ncol;
+  const size_t cs = S.mixed ? sizeof(float2) : sizeof(double2);
+  int64_t held = 0;
+  for (int i = 1; i <= 8; i++) {
+    HIPCHK(hipMalloc(&S.F[i], nf * sizeof(double2)));
+    HIPCHK(hipMemset(S.F[i], 0, nf * sizeof(double2)));
+    held += nf * sizeof(double2);
+  }
+  HIPCHK(hipMalloc(&S.P, 6 * nf * cs));
+  HIPCHK(hipMemset(S.P, 0, 6 * nf * cs));
+  held += 6 * nf * cs;
+  // solver workspace: two per-column sweeps (48 B per row each) or four D0 sweeps (16 B)
+  const size_t rows = (size_t)(D.ny - 2) * D.ncol;
+  S.ws_bytes = std::max(2 * rows * 48, nf * sizeof(double2));
+  HIPCHK(hipMalloc(&S.ws, S.ws_bytes));
+  held += S.ws_bytes;
+  char *w = (char *)S.ws;
+  for (int k = 0; k < 2; k++) {
+    char *b = w + k * rows * 48;
+    S.pws[k].U1 = (double *)b;
+    S.pws[k].U2 = (double *)(b + rows * 8);
+    S.pws[k].B1 = (double *)(b + rows * 16);
+    S.pws[k].B2 = (double *)(b + rows * 24);
+    S.pws[k].XR = (double2 *)(b + rows * 32);
+  }
+  for (int k = 0; k < 4; k++) S.XR[k] = (double2 *)(w + k * rows * 16);
+  // transform chunk: rows of the physical-space buffers Hv (3 fields) and Hp (6)
+  const size_t per_row = 9 * (size_t)D.nzd * D.NXp * cs;
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(hipMemGetInfo(&free_b, &total_b));
+  int chunk = D.NYg;
+  if (g_chunk_request > 0) chunk = std::min(g_chunk_request, D.NYg);
+  else {
+    const size_t budget = free_b > (size_t)4 << 30 ? free_b - ((size_t)4 << 30) : free_b / 2;
+    if ((size_t)chunk * per_row > budget) chunk = (int)std::max<size_t>(1, budget / per_row);
+  }
+  S.chunk = chunk;
+  HIPCHK(hipMalloc(&S.Hv, 3 * (size_t)chunk * D.nzd * D.NXp * cs));
+  HIPCHK(hipMalloc(&S.Hp, 6 * (size_t)chunk * D.nzd * D.NXp * cs));
+  held += 9 * (size_t)chunk * D.nzd * D.NXp * cs;
+  // device tables
+  std::vector<double> derd((size_t)D.ny * 20);
+  for (int iy = 0; iy < D.ny; iy++)
+    for (int n = 0; n < 4; n++)
+      for (int j = 0; j < 5; j++) derd[(size_t)iy * 20 + n * 5 + j] = S.der[(size_t)iy + (size_t)D.ny * (n + 4 * j)];
+  HIPCHK(hipMalloc(&S.d_der, derd.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(S.d_der, derd.data(), derd.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&S.d_y, S.y.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(S.d_y, S.y.data(), S.y.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&S.d_dyl, S.dyl.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(S.d_dyl, S.dyl.data(), S.dyl.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&S.d_f0, S.f0.rows.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(S.d_f0, S.f0.rows.data(), S.f0.rows.size() * sizeof(double), hipMemcpyHostToDevice));
+  S.part_cap = std::max<size_t>((size_t)D.NYg * D.nzd, (size_t)((D.ncol + 255) / 256) * 9 + 64);
+  HIPCHK(hipMalloc(&S.d_part, S.part_cap * sizeof(double)));
+  if (make_fft_tables(D, S.mixed, &S.tables)) return fail("twiddle tables");
+  S.bytes_held = held;
+  S.live = true;
+  return 0;
+}
+
+int hst_gpu_free(void) {
+  release();
+  return 0;
+}
+
+int hst_gpu_sync(void) {
+  if (int rc = check_live()) return rc;
+  HIPCHK(hipStreamSynchronize(S.stream));
+  return 0;
+}
+
+int hst_gpu_grid(hst_grid_info *info, double *y, double *dyl, double *der) {
+  if (int rc = check_live()) return rc;
+  if (info) *info = S.g;
+  if (y) std::memcpy(y, S.y.data(), S.y.size() * sizeof(double));
+  if (dyl) std::memcpy(dyl, S.dyl.data(), S.dyl.size() * sizeof(double));
+  if (der) std::memcpy(der, S.der.data(), S.der.size() * sizeof(double));
+  return 0;
+}
+
+int hst_gpu_upload_field(int field, const double *host) {
+  if (int rc = check_live()) return rc;
+  if (int rc = check_field(field)) return rc;
+  const size_t nf = (size_t)S.D.NYg * S.D.ncol;
+  double2 *stage = (double2 *)S.ws;
+  HIPCHK(hipMemcpyAsync(stage, host, nf * sizeof(double2), hipMemcpyHostToDevice, S.stream));
+  int rc = launch_to_device_layout(S.D, stage, S.F[field], S.stream);
+  if (rc) return hip_fail("layout", (hipError_t)rc);
+  HIPCHK(hipStreamSynchronize(S.stream));
+  return 0;
+}
+
+int hst_gpu_download_field(int field, double *host) {
+  if (int rc = check_live()) return rc;
+  if (int rc = check_field(field)) return rc;
+  const size_t nf = (size_t)S.D.NYg * S.D.ncol;
+  double2 *stage = (double2 *)S.ws;
+  int rc = launch_to_ref_layout(S.D, S.F[field], stage, S.stream);
+  if (rc) return hip_fail("layout", (hipError_t)rc);
+  HIPCHK(hipMemcpyAsync(host, stage, nf * sizeof(double2), hipMemcpyDeviceToHost, S.stream));
+  HIPCHK(hipStreamSynchronize(S.stream));
+  return 0;
+}
+
+int hst_gpu_zero_field(int field) {
+  if (int rc = check_live()) return rc;
+  if (int rc = check_field(field)) return rc;
+  HIPCHK(hipMemsetAsync(S.F[field], 0, (size_t)S.D.NYg * S.D.ncol * sizeof(double2), S.stream));
+  return 0;
+}
+
+int hst_gpu_fill_ghosts(int field, double shift_x, double shift_z) {
+  if (int rc = check_live()) return rc;
+  if (int rc = check_field(field)) return rc;
+  double2 *f[1] = {S.F[field]};
+  return ghosts(1, f, shift_x, shift_z);
+}
+
+int hst_gpu_nonlinear(double *cfl) {
+  if (int rc = check_live()) return rc;
+  return nonlinear_pass(0, cfl != nullptr, cfl);
+}
+
+int hst_gpu_compute_cfl(double *cfl) {
+  if (int rc = check_live()) return rc;
+  double c = 0.0;
+  int rc = nonlinear_pass(2, true, &c);
+  if (cfl) *cfl = c;
+  return rc;
+}
+
+int hst_gpu_roundtrip(void) {
+  if (int rc = check_live()) return rc;
+  return nonlinear_pass(1, false, nullptr);
+}
+
+int hst_gpu_product_to_field(int p, int field) {
+  if (int rc = check_live()) return rc;
+  if (int rc = check_field(field)) return rc;
+  if (p < 1 || p > 6) return fail("product index out of range (1..6)");
+  int rc = launch_product_to_field(S.D, S.mixed, P_of(p - 1), S.F[field], S.stream);
+  if (rc) return hip_fail("product_to_field", (hipError_t)rc);
+  return 0;
+}
+
+int hst_gpu_buildrhs_prepare(const double ode[3], double deltat, double time) {
+  if (int rc = check_live()) return rc;
+  return buildrhs(ode, deltat, time, 0, false, 0.0, 0.0);
+}
+
+int hst_gpu_buildrhs(const double ode[3], int g) {
+  if (int rc = check_live()) return rc;
+  if (g != 1 && g != 2) return fail("group must be 1 or 2");
+  return buildrhs(ode, 1.0, 0.0, g, false, 0.0, 0.0);
+}
+
+int hst_gpu_shear_shift(double dt_sub, double time) {
+  if (int rc = check_live()) return rc;
+  return shear_shift(dt_sub, time);
+}
+
+int hst_gpu_line_solve(int kind, double lambda, int src, int dst, double shift_x, double shift_z) {
+  if (int rc = check_live()) return rc;
+  if (int rc = check_field(src)) return rc;
+  if (int rc = check_field(dst)) return rc;
+  if (src == dst) return fail("line_solve: src and dst must be different fields");
+  if (kind == HST_KIND_D0 || kind == HST_KIND_DY) {
+    return run(kind == HST_KIND_D0 ? "d0_solve" : "dy_solve", 4.0 * bytes_field(), [&] {
+      return launch_d0solve(S.D, kind == HST_KIND_DY ? 1 : 0, 0, S.F[src], S.F[dst], nullptr, S.XR[0], S.d_f0, S.fc,
+                            S.d_der, S.d_dyl, shift_x, shift_z, nullptr, nullptr, S.stream);
+    });
+  }
+  if (kind < 1 || kind > 3) return fail("unknown system kind");
+  const int kinds[1] = {kind};
+  const double2 *s[1] = {S.F[src]};
+  double2 *d[1] = {S.F[dst]};
+  return run("penta", 8.0 * bytes_field(), [&] {
+    return launch_penta(S.D, 1, kinds, lambda, s, d, S.pws, S.d_der, shift_x, shift_z, S.stream);
+  });
+}
+
+int hst_gpu_linsolve(double lambda, double time) {
+  if (int rc = check_live()) return rc;
+  return linsolve(lambda, time);
+}
+
+int hst_gpu_timestep(double deltat, double *time, double *cfl) {
+  if (int rc = check_live()) return rc;
+  if (!time) return fail("time is NULL");
+  double t = *time, c = 0.0;
+  for (int i = 0; i < 3; i++) {
+    const double *ode = RK[i];
+    int rc = 0;
+    const bool want = (i == 2) && cfl != nullptr;
+    if (!S.deck.linear) rc = nonlinear_pass(0, want, &c);
+    else if (want) rc = nonlinear_pass(2, true, &c);  // linear: the transform only for the CFL number
+    if (rc) return rc;
+    const double dt_sub = 2.0 / ode[0] * deltat;
+    const bool fuse_shift = !S.deck.exact_shift;
+    const double dgx = S.deck.S * dt_sub, dgz = s2_integral(t, t + dt_sub);
+    rc = buildrhs(ode, deltat, t, S.deck.linear ? 0 : 3, fuse_shift, dgx, dgz);
+    if (rc) return rc;
+    if (S.deck.exact_shift && (rc = exact_shift(dt_sub, t))) return rc;
+    t = t + dt_sub;
+    if ((rc = linsolve(ode[0] / deltat, t))) return rc;
+  }
+  *time = t;
+  if (cfl) *cfl = c;
+  return 0;
+}
+
+int hst_gpu_outstats(double time, double out[10]) {
+  if (int rc = check_live()) return rc;
+  int nb = 0;
+  int rc = run("stats", 3.0 * bytes_field(), [&] {
+    return launch_stats(S.D, S.F[1], S.F[2], S.F[3], S.d_dyl, S.d_part, &nb, S.stream);
+  });
+  if (rc) return rc;
+  std::vector<double> h((size_t)nb * 9);
+  HIPCHK(hipMemcpyAsync(h.data(), S.d_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, S.stream));
+  HIPCHK(hipStreamSynchronize(S.stream));
+  double s[9] = {0};
+  for (int b = 0; b < nb; b++)
+    for (int k = 0; k < 9; k++) s[k] += h[(size_t)b * 9 + k];
+  double sx, sz;
+  shear_shifts(time, sx, sz);
+  double eps = s[6];
+  for (int c = 1; c <= 3; c++) {
+    int np = 0;
+    rc = run("dy_stats", 3.0 * bytes_field(), [&] {
+      return launch_d0solve(S.D, 1, 2, S.F[c], nullptr, nullptr, S.XR[0], S.d_f0, S.fc, S.d_der, S.d_dyl, sx, sz,
+                            S.d_part, &np, S.stream);
+    });
+    if (rc) return rc;
+    std::vector<double> g((size_t)np);
+    HIPCHK(hipMemcpyAsync(g.data(), S.d_part, g.size() * sizeof(double), hipMemcpyDeviceToHost, S.stream));
+    HIPCHK(hipStreamSynchronize(S.stream));
+    for (double v : g) eps += v;
+  }
+  double uu = s[0], vv = s[1], ww = s[2], uv = s[3], vw = s[4], uw = s[5], mfx = s[7], mfz = s[8];
+  double ly_w = 0.0;
+  for (double d : S.dyl) ly_w += d;
+  uu = uu - mfx * mfx / ly_w;
+  ww = ww - mfz * mfz / ly_w;
+  uw = uw - mfx * mfz / ly_w;
+  const double q2 = uu + vv + ww, inv_ly = 1.0 / S.deck.ly;
+  out[0] = 0.5 * q2 * inv_ly;
+  out[1] = S.g.ni * eps * inv_ly;
+  out[2] = uv * inv_ly;
+  out[3] = vw * inv_ly;
+  out[4] = uu * inv_ly;
+  out[5] = vv * inv_ly;
+  out[6] = ww * inv_ly;
+  out[7] = uw * inv_ly;
+  out[8] = mfx;
+  out[9] = mfz;
+  return 0;
+}
+
+int hst_gpu_profile(int enable) {
+  if (int rc = check_live()) return rc;
+  if (enable) {
+    flush_profile();
+    S.table.clear();
+    S.order.clear();
+    S.prof = true;
+  } else {
+    flush_profile();
+    S.prof = false;
+  }
+  return 0;
+}
+
+int hst_gpu_profile_count(void) {
+  flush_profile();
+  return (int)S.order.size();
+}
+
+int hst_gpu_profile_entry(int i, char *name, int len, int64_t *launches, double *ms, double *bytes) {
+  flush_profile();
+  if (i < 0 || i >= (int)S.order.size()) return fail("profile entry out of range");
+  const ProfEntry &e = S.table[S.order[i]];
+  if (name && len > 0) std::snprintf(name, (size_t)len, "%s", S.order[i].c_str());
+  if (launches) *launches = e.launches;
+  if (ms) *ms = e.ms;
+  if (bytes) *bytes = e.bytes;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,47 @@
+// Complex numbers of the kernels: a pair (x = real, y = imaginary) of float
+// or double with the operations the transforms and the stencils need.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace hst {
+
+template <typename R>
+struct alignas(2 * sizeof(R)) Cx {
+  R x, y;
+};
+
+#define HSTD __host__ __device__ __forceinline__
+
+template <typename R> HSTD Cx<R> operator+(Cx<R> a, Cx<R> b) { return Cx<R>{a.x + b.x, a.y + b.y}; }
+template <typename R> HSTD Cx<R> operator-(Cx<R> a, Cx<R> b) { return Cx<R>{a.x - b.x, a.y - b.y}; }
+template <typename R> HSTD Cx<R> operator-(Cx<R> a) { return Cx<R>{-a.x, -a.y}; }
+template <typename R> HSTD Cx<R> operator*(Cx<R> a, Cx<R> b) {
+  return Cx<R>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R> HSTD Cx<R> operator*(Cx<R> a, R s) { return Cx<R>{a.x * s, a.y * s}; }
+template <typename R> HSTD Cx<R> operator*(R s, Cx<R> a) { return Cx<R>{s * a.x, s * a.y}; }
+template <typename R> HSTD Cx<R> conj(Cx<R> a) { return Cx<R>{a.x, -a.y}; }
+template <typename R> HSTD Cx<R> mul_i(Cx<R> a) { return Cx<R>{-a.y, a.x}; }    // i a
+template <typename R> HSTD Cx<R> mul_mi(Cx<R> a) { return Cx<R>{a.y, -a.x}; }   // -i a
+// (i b) a for real b: the product of the purely imaginary i b with a
+template <typename R> HSTD Cx<R> imul(R b, Cx<R> a) { return Cx<R>{-(b * a.y), b * a.x}; }
+template <typename R> HSTD Cx<R> cdiv(Cx<R> a, Cx<R> b) {
+  // Smith's algorithm
+  if (fabs(b.x) >= fabs(b.y)) {
+    R r = b.y / b.x, d = b.x + r * b.y;
+    return Cx<R>{(a.x + a.y * r) / d, (a.y - a.x * r) / d};
+  }
+  R r = b.x / b.y, d = b.y + r * b.x;
+  return Cx<R>{(a.x * r + a.y) / d, (a.y * r - a.x) / d};
+}
+
+template <typename R> HSTD Cx<R> cx_from(double2 v) { return Cx<R>{(R)v.x, (R)v.y}; }
+HSTD Cx<double> ld(const double2 *p) { double2 v = *p; return Cx<double>{v.x, v.y}; }
+HSTD void st(double2 *p, Cx<double> v) { *p = make_double2(v.x, v.y); }
+HSTD Cx<double> ld(const float2 *p) { float2 v = *p; return Cx<double>{(double)v.x, (double)v.y}; }
+HSTD Cx<float> ldf(const float2 *p) { float2 v = *p; return Cx<float>{v.x, v.y}; }
+
+#undef HSTD
+
+}  // namespace hst

@@ -1,0 +1,501 @@
+// The spectral <-> physical round trip of the RK3 substep on gfx950:
+//
+//   k_zift   z inverse transform of u, v, w (3/2 padding fused into the load),
+//            written transposed to x-fastest lines (Hv)
+//   k_xpass  per (row, z point): x inverse real transform of u, v, w, the six
+//            products (and the CFL number), x forward real transforms of the
+//            products truncated to the nx+1 kept modes (Hp)
+//   k_zfft   z forward transform of the six products, truncated to the 2nz+1
+//            kept modes, written to the spectral layout (P)
+//
+// Every transform is an in-LDS Stockham FFT of radices 8, 4, 2 and 3 with the
+// butterflies held in registers between the two barriers of a stage.  Real
+// transforms of length 2M run as complex transforms of length M.  Field
+// layout on the device: [row][ix][iz] with iz fastest (a Fourier column is
+// one (ix, iz) pair; col = ix*(2nz+1) + iz + nz), rows -2..ny+1.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "hst_cx.h"
+#include "hst_internal.h"
+
+namespace hst {
+
+// ------------------------------------------------------------------ radix plan
+
+__host__ __device__ constexpr int pow2_part(int N) { return N % 3 == 0 ? N / 3 : N; }
+__host__ __device__ constexpr int n_radix8(int N) {
+  int m = pow2_part(N), c = 0;
+  while (m % 8 == 0) { m /= 8; c++; }
+  return c;
+}
+__host__ __device__ constexpr int radix_rem(int N) {
+  int m = pow2_part(N);
+  while (m % 8 == 0) m /= 8;
+  return m;  // 1, 2 or 4
+}
+__host__ __device__ constexpr int n_stages(int N) { return (N % 3 == 0 ? 1 : 0) + n_radix8(N) + (radix_rem(N) > 1 ? 1 : 0); }
+__host__ __device__ constexpr int radix_at(int N, int s) {
+  return (N % 3 == 0) ? (s == 0 ? 3 : (s - 1 < n_radix8(N) ? 8 : radix_rem(N))) : (s < n_radix8(N) ? 8 : radix_rem(N));
+}
+// threads per line
+__host__ __device__ constexpr int fft_threads(int N) { return N >= 16 ? (N / 8 > 192 ? 192 : N / 8) : 1; }
+
+// ------------------------------------------------------------------ small DFTs
+
+template <typename R, int SIGN>
+__device__ __forceinline__ void dft2(Cx<R> *v) {
+  Cx<R> a = v[0], b = v[1];
+  v[0] = a + b;
+  v[1] = a - b;
+}
+
+template <typename R, int SIGN>
+__device__ __forceinline__ void dft3(Cx<R> *v) {
+  const R h = (R)0.86602540378443864676;  // sin(2 pi/3)
+  Cx<R> t1 = v[1] + v[2];
+  Cx<R> t2 = v[0] - t1 * (R)0.5;
+  Cx<R> d = (v[1] - v[2]) * (h * (R)SIGN);
+  v[0] = v[0] + t1;
+  v[1] = t2 + mul_i(d);
+  v[2] = t2 - mul_i(d);
+}
+
+template <typename R, int SIGN>
+__device__ __forceinline__ void dft4(Cx<R> *v) {
+  Cx<R> a = v[0] + v[2], b = v[0] - v[2], c = v[1] + v[3], d = (v[1] - v[3]);
+  d = SIGN > 0 ? mul_i(d) : mul_mi(d);
+  v[0] = a + c;
+  v[2] = a - c;
+  v[1] = b + d;
+  v[3] = b - d;
+}
+
+template <typename R, int SIGN>
+__device__ __forceinline__ void dft8(Cx<R> *v) {
+  const R r2 = (R)0.70710678118654752440;
+  Cx<R> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<R, SIGN>(e);
+  dft4<R, SIGN>(o);
+  // o[k] *= exp(SIGN 2 pi i k / 8)
+  Cx<R> o1 = o[1], o3 = o[3];
+  if (SIGN > 0) {
+    o[1] = Cx<R>{(o1.x - o1.y) * r2, (o1.x + o1.y) * r2};
+    o[2] = mul_i(o[2]);
+    o[3] = Cx<R>{(-o3.x - o3.y) * r2, (o3.x - o3.y) * r2};
+  } else {
+    o[1] = Cx<R>{(o1.x + o1.y) * r2, (o1.y - o1.x) * r2};
+    o[2] = mul_mi(o[2]);
+    o[3] = Cx<R>{(-o3.x + o3.y) * r2, (-o3.x - o3.y) * r2};
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    v[k] = e[k] + o[k];
+    v[k + 4] = e[k] - o[k];
+  }
+}
+
+template <typename R, int SIGN, int RAD>
+__device__ __forceinline__ void dft(Cx<R> *v) {
+  if constexpr (RAD == 2) dft2<R, SIGN>(v);
+  else if constexpr (RAD == 3) dft3<R, SIGN>(v);
+  else if constexpr (RAD == 4) dft4<R, SIGN>(v);
+  else dft8<R, SIGN>(v);
+}
+
+// ------------------------------------------------------------------ in-LDS FFT
+
+// One Stockham stage (decimation in frequency): sequences of the current
+// length len = N/s interleaved with stride s; butterfly (q, n1) reads
+// x[q + s (n1 + L n2)], n2 < r, and writes its DFT times w_len^(n1 k2) to
+// y[q + s k2 + s r n1].  In place: every thread holds its butterflies in
+// registers across the barrier.
+template <typename R, int N, int T, int SIGN, int RAD, int S>
+__device__ __forceinline__ void fft_stage(Cx<R> *line, int t, const Cx<R> *__restrict__ tw) {
+  constexpr int len = N / S, L = len / RAD, B = N / RAD, K = (B + T - 1) / T;
+  Cx<R> v[K][RAD];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int b = t + T * k;
+    if (B % T == 0 || b < B) {
+      const int q = b % S, n1 = b / S;
+#pragma unroll
+      for (int n2 = 0; n2 < RAD; n2++) v[k][n2] = line[q + S * (n1 + L * n2)];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int b = t + T * k;
+    if (B % T == 0 || b < B) {
+      const int q = b % S, n1 = b / S;
+      dft<R, SIGN, RAD>(v[k]);
+      line[q + S * RAD * n1] = v[k][0];
+#pragma unroll
+      for (int k2 = 1; k2 < RAD; k2++) {
+        Cx<R> w = tw[n1 * k2 * S];
+        if (SIGN > 0) w.y = -w.y;
+        line[q + S * k2 + S * RAD * n1] = v[k][k2] * w;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <typename R, int N, int T, int SIGN, int ST = 0, int S = 1>
+__device__ __forceinline__ void fft_line(Cx<R> *line, int t, const Cx<R> *__restrict__ tw) {
+  if constexpr (ST < n_stages(N)) {
+    constexpr int RAD = radix_at(N, ST);
+    fft_stage<R, N, T, SIGN, RAD, S>(line, t, tw);
+    fft_line<R, N, T, SIGN, ST + 1, S * RAD>(line, t, tw);
+  }
+}
+
+// ------------------------------------------------------------------ tables
+
+template <typename R>
+struct Tables {
+  Cx<R> *twz;   // exp(-2 pi i k / nzd), k < nzd
+  Cx<R> *twx;   // exp(-2 pi i k / M), k < M = nxd
+  Cx<R> *twh;   // exp(-i pi k / M), k < M
+};
+
+struct TableSet {
+  int mixed;
+  Tables<double> d;
+  Tables<float> f;
+};
+
+template <typename R>
+static int fill_table(Cx<R> **dst, int n, double scale_den) {
+  std::vector<Cx<R>> h(n);
+  for (int k = 0; k < n; k++) {
+    const double a = -2.0 * M_PI * (double)k / scale_den;
+    h[k] = Cx<R>{(R)std::cos(a), (R)std::sin(a)};
+  }
+  if (hipMalloc(dst, sizeof(Cx<R>) * n) != hipSuccess) return 1;
+  return hipMemcpy(*dst, h.data(), sizeof(Cx<R>) * n, hipMemcpyHostToDevice) != hipSuccess;
+}
+
+bool fft_size_supported(int n) {
+  switch (n) {
+    case 6: case 12: case 24: case 48: case 64: case 96: case 128: case 192: case 256:
+    case 384: case 512: case 768: case 1024: case 1536:
+      return true;
+    default:
+      return false;
+  }
+}
+
+int make_fft_tables(const Dims &D, int mixed, void **out) {
+  TableSet *t = new TableSet();
+  t->mixed = mixed;
+  int rc = 0;
+  if (!mixed) {
+    rc |= fill_table(&t->d.twz, D.nzd, D.nzd);
+    rc |= fill_table(&t->d.twx, D.nxd, D.nxd);
+    rc |= fill_table(&t->d.twh, D.nxd, 2.0 * D.nxd);
+  } else {
+    rc |= fill_table(&t->f.twz, D.nzd, D.nzd);
+    rc |= fill_table(&t->f.twx, D.nxd, D.nxd);
+    rc |= fill_table(&t->f.twh, D.nxd, 2.0 * D.nxd);
+  }
+  *out = t;
+  return rc;
+}
+
+void free_fft_tables(void *p) {
+  TableSet *t = (TableSet *)p;
+  if (!t) return;
+  if (!t->mixed) { (void)hipFree(t->d.twz); (void)hipFree(t->d.twx); (void)hipFree(t->d.twh); }
+  else { (void)hipFree(t->f.twz); (void)hipFree(t->f.twx); (void)hipFree(t->f.twh); }
+  delete t;
+}
+
+// ------------------------------------------------------------------ launch shapes
+
+template <typename R, int N>
+struct ZCfg {
+  static constexpr int T = fft_threads(N);
+  static constexpr int LDS_MAX = 100 * 1024;
+  static constexpr int G0 = (sizeof(R) == 8 ? 8 : 16);  // 128-byte transposed segments
+  static constexpr int g_lds(int g) { return (g > 1 && g * (N + 1) * (int)sizeof(Cx<R>) > LDS_MAX) ? g_lds(g / 2) : g; }
+  static constexpr int g_thr(int g) { return (g > 1 && g * T > 1024) ? g_thr(g / 2) : g; }
+  static constexpr int G = g_thr(g_lds(G0));
+  static constexpr int NT = G * T;
+};
+
+template <typename R, int M>
+struct XCfg {
+  static constexpr int T = fft_threads(M);
+  static constexpr int LDS_MAX = 100 * 1024;
+  static constexpr int per_line = 4 * (M + 1) * (int)sizeof(Cx<R>);
+  static constexpr int g_fit(int g) { return (g > 1 && (g * per_line > LDS_MAX || g * T > 512)) ? g_fit(g / 2) : g; }
+  static constexpr int G = g_fit(4);
+  static constexpr int NT = G * T;
+};
+
+// ------------------------------------------------------------------ k_zift
+
+// z inverse transform of the three velocity components for the rows of a
+// chunk.  grid (x tiles of G modes, rows, component); output Hv[c][row][zp][ix].
+template <typename R, int NZD>
+__global__ void __launch_bounds__((ZCfg<R, NZD>::NT))
+    k_zift(const double2 *__restrict__ V1, const double2 *__restrict__ V2, const double2 *__restrict__ V3,
+           Cx<R> *__restrict__ Hv, Dims D, int row0, int nrows, const Cx<R> *__restrict__ tw) {
+  constexpr int G = ZCfg<R, NZD>::G, T = ZCfg<R, NZD>::T, LS = NZD + 1;
+  __shared__ Cx<R> lds[G * LS];
+  const int c = blockIdx.z, rowc = blockIdx.y, row = row0 + rowc, ix0 = blockIdx.x * G;
+  const double2 *src = (c == 0 ? V1 : (c == 1 ? V2 : V3)) + (size_t)row * D.ncol + (size_t)ix0 * D.NZm;
+  const int nz = D.nz;
+  for (int j = 0; j < G; j++) {
+    const bool ok = ix0 + j < D.NXm;
+    for (int k = threadIdx.x; k < NZD; k += blockDim.x) {
+      Cx<R> v{(R)0, (R)0};
+      if (ok) {
+        if (k <= nz) v = cx_from<R>(src[(size_t)j * D.NZm + k + nz]);
+        else if (k >= NZD - nz) v = cx_from<R>(src[(size_t)j * D.NZm + (k - NZD) + nz]);
+      }
+      lds[j * LS + k] = v;
+    }
+  }
+  __syncthreads();
+  const int g = threadIdx.x / T, t = threadIdx.x % T;
+  fft_line<R, NZD, T, +1>(lds + g * LS, t, tw);
+  Cx<R> *dst = Hv + ((size_t)c * nrows + rowc) * NZD * D.NXp;
+  for (int e = threadIdx.x; e < G * NZD; e += blockDim.x) {
+    const int j = e % G, zp = e / G;
+    if (ix0 + j < D.NXm) dst[(size_t)zp * D.NXp + ix0 + j] = lds[j * LS + zp];
+  }
+}
+
+// ------------------------------------------------------------------ k_zfft
+
+template <typename R, int NZD>
+__global__ void __launch_bounds__((ZCfg<R, NZD>::NT))
+    k_zfft(const Cx<R> *__restrict__ Hp, Cx<R> *__restrict__ P0, size_t pstride, Dims D, int row0, int nrows,
+           const Cx<R> *__restrict__ tw) {
+  constexpr int G = ZCfg<R, NZD>::G, T = ZCfg<R, NZD>::T, LS = NZD + 1;
+  __shared__ Cx<R> lds[G * LS];
+  const int p = blockIdx.z, rowc = blockIdx.y, row = row0 + rowc, ix0 = blockIdx.x * G;
+  const Cx<R> *src = Hp + ((size_t)p * nrows + rowc) * NZD * D.NXp;
+  for (int e = threadIdx.x; e < G * NZD; e += blockDim.x) {
+    const int j = e % G, zp = e / G;
+    lds[j * LS + zp] = (ix0 + j < D.NXm) ? src[(size_t)zp * D.NXp + ix0 + j] : Cx<R>{(R)0, (R)0};
+  }
+  __syncthreads();
+  const int g = threadIdx.x / T, t = threadIdx.x % T;
+  fft_line<R, NZD, T, -1>(lds + g * LS, t, tw);
+  Cx<R> *dst = P0 + (size_t)p * pstride + (size_t)row * D.ncol + (size_t)ix0 * D.NZm;
+  const int nz = D.nz;
+  for (int j = 0; j < G; j++) {
+    if (ix0 + j >= D.NXm) break;
+    for (int izm = threadIdx.x; izm < D.NZm; izm += blockDim.x) {
+      const int iz = izm - nz;
+      dst[(size_t)j * D.NZm + izm] = lds[j * LS + (iz >= 0 ? iz : NZD + iz)];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ k_xpass
+
+// One (row, z point) line per group of T threads, G lines per block.
+// MODE 0: six products; 1: the round trip (u, v, w times factor); 2: CFL only.
+template <typename R, int M, int MODE>
+__global__ void __launch_bounds__((XCfg<R, M>::NT))
+    k_xpass(const Cx<R> *__restrict__ Hv, Cx<R> *__restrict__ Hp, Dims D, int row0, int nrows,
+            const Cx<R> *__restrict__ twx, const Cx<R> *__restrict__ twh, const double *__restrict__ dyl,
+            double *__restrict__ cfl_part, int want_cfl) {
+  constexpr int G = XCfg<R, M>::G, T = XCfg<R, M>::T, LS = M + 1;
+  __shared__ Cx<R> vel[3 * G * LS];
+  __shared__ Cx<R> work[G * LS];
+  __shared__ double red[XCfg<R, M>::NT / 64 + 1];
+  const int g = threadIdx.x / T, t = threadIdx.x % T;
+  const int rowc = blockIdx.y, row = row0 + rowc;
+  const int zp = blockIdx.x * G + g;  // nzd is a multiple of G (checked by the launcher)
+  const int nx = D.nx, NZD = D.nzd;
+  Cx<R> *wl = work + g * LS;
+  // X -> Z of the half-length complex inverse transform, per component
+  for (int c = 0; c < 3; c++) {
+    const Cx<R> *src = Hv + (((size_t)c * nrows + rowc) * NZD + zp) * D.NXp;
+    for (int k = t; k <= nx; k += T) wl[k] = src[k];
+    __syncthreads();
+    Cx<R> *vl = vel + (c * G + g) * LS;
+    for (int k = t; k < M; k += T) {
+      Cx<R> a = (k <= nx) ? wl[k] : Cx<R>{(R)0, (R)0};
+      if (k == 0) a.y = (R)0;  // c2r: the imaginary part of the kx = 0 bin is ignored
+      const int j = M - k;
+      Cx<R> b = (k > 0 && j <= nx) ? conj(wl[j]) : Cx<R>{(R)0, (R)0};
+      Cx<R> e = conj(twh[k]);  // exp(+i pi k / M)
+      vl[k] = (a + b) + mul_i(e * (a - b));
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) fft_line<R, M, T, +1>(vel + (c * G + g) * LS, t, twx);
+  // vel[c][m] = x[2m] + i x[2m+1]
+  const Cx<R> *u = vel + (0 * G + g) * LS, *v = vel + (1 * G + g) * LS, *w = vel + (2 * G + g) * LS;
+  if (want_cfl) {
+    double cm = 0.0;
+    const int iy = row - 2;
+    if (iy >= 0 && iy < D.ny) {
+      const double idx = 1.0 / D.dx, idy = 1.0 / dyl[iy], idz = 1.0 / D.dz;
+      for (int m = t; m < M; m += T) {
+        double a = fabs((double)u[m].x) * idx + fabs((double)v[m].x) * idy + fabs((double)w[m].x) * idz;
+        double b = fabs((double)u[m].y) * idx + fabs((double)v[m].y) * idy + fabs((double)w[m].y) * idz;
+        cm = fmax(cm, fmax(a, b));
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) cm = fmax(cm, __shfl_xor(cm, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double r = 0.0;
+      for (int i = 0; i < (int)((blockDim.x + 63) >> 6); i++) r = fmax(r, red[i]);
+      cfl_part[blockIdx.y * gridDim.x + blockIdx.x] = r;
+    }
+  }
+  if constexpr (MODE == 2) return;
+  const R factor = (R)D.factor;
+  constexpr int NP = (MODE == 0 ? 6 : 3);
+  for (int p = 0; p < NP; p++) {
+    for (int m = t; m < M; m += T) {
+      Cx<R> r;
+      if constexpr (MODE == 1) {
+        const Cx<R> a = (p == 0 ? u[m] : (p == 1 ? v[m] : w[m]));
+        r = Cx<R>{a.x * factor, a.y * factor};
+      } else {
+        const Cx<R> a = (p == 0 || p == 3 || p == 5) ? u[m] : (p == 1 || p == 4 ? v[m] : w[m]);
+        const Cx<R> b = (p == 0) ? u[m] : (p == 1 || p == 3) ? v[m] : w[m];
+        // uu vv ww uv vw uw, each (a b) factor on the even and the odd points
+        r = Cx<R>{a.x * b.x * factor, a.y * b.y * factor};
+      }
+      wl[m] = r;
+    }
+    __syncthreads();
+    fft_line<R, M, T, -1>(wl, t, twx);
+    Cx<R> *dst = Hp + (((size_t)p * nrows + rowc) * NZD + zp) * D.NXp;
+    for (int k = t; k <= nx; k += T) {
+      const Cx<R> Z = wl[k], Zc = conj(wl[(M - k) % M]);
+      const Cx<R> e = twh[k];  // exp(-i pi k / M)
+      const Cx<R> s = (Z + Zc) * (R)0.5, d = (Z - Zc) * (R)0.5;
+      dst[k] = s - mul_i(e * d);
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+
+template <typename R, int NZD>
+static int run_zift(const Dims &D, const double2 *const V[3], void *Hv, int row0, int nrows, const Tables<R> &tb,
+                    hipStream_t s) {
+  typedef ZCfg<R, NZD> C;
+  dim3 grid((D.NXm + C::G - 1) / C::G, nrows, 3);
+  hipLaunchKernelGGL((k_zift<R, NZD>), grid, dim3(C::NT), 0, s, V[0], V[1], V[2], (Cx<R> *)Hv, D, row0, nrows,
+                     tb.twz);
+  return hipGetLastError();
+}
+
+template <typename R, int NZD>
+static int run_zfft(const Dims &D, int np, const void *Hp, void *P0, size_t pstride, int row0, int nrows,
+                    const Tables<R> &tb, hipStream_t s) {
+  typedef ZCfg<R, NZD> C;
+  dim3 grid((D.NXm + C::G - 1) / C::G, nrows, np);
+  hipLaunchKernelGGL((k_zfft<R, NZD>), grid, dim3(C::NT), 0, s, (const Cx<R> *)Hp, (Cx<R> *)P0, pstride, D, row0,
+                     nrows, tb.twz);
+  return hipGetLastError();
+}
+
+template <typename R, int M>
+static int run_xpass(const Dims &D, int mode, const void *Hv, void *Hp, int row0, int nrows, const Tables<R> &tb,
+                     const double *dyl, double *cfl_part, int want_cfl, int *n_part, hipStream_t s) {
+  typedef XCfg<R, M> C;
+  if (D.nzd % C::G) return hipErrorInvalidValue;
+  dim3 grid(D.nzd / C::G, nrows);
+  if (n_part) *n_part = grid.x * grid.y;
+  if (mode == 0)
+    hipLaunchKernelGGL((k_xpass<R, M, 0>), grid, dim3(C::NT), 0, s, (const Cx<R> *)Hv, (Cx<R> *)Hp, D, row0, nrows,
+                       tb.twx, tb.twh, dyl, cfl_part, want_cfl);
+  else if (mode == 1)
+    hipLaunchKernelGGL((k_xpass<R, M, 1>), grid, dim3(C::NT), 0, s, (const Cx<R> *)Hv, (Cx<R> *)Hp, D, row0, nrows,
+                       tb.twx, tb.twh, dyl, cfl_part, want_cfl);
+  else
+    hipLaunchKernelGGL((k_xpass<R, M, 2>), grid, dim3(C::NT), 0, s, (const Cx<R> *)Hv, (Cx<R> *)Hp, D, row0, nrows,
+                       tb.twx, tb.twh, dyl, cfl_part, 1);
+  return hipGetLastError();
+}
+
+#define HST_FOR_SIZES(X) X(6) X(12) X(24) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536)
+
+template <typename R>
+static int zift_dispatch(const Dims &D, const double2 *const V[3], void *Hv, int row0, int nrows, const Tables<R> &tb,
+                         hipStream_t s) {
+  switch (D.nzd) {
+#define CASE(n) case n: return run_zift<R, n>(D, V, Hv, row0, nrows, tb, s);
+    HST_FOR_SIZES(CASE)
+#undef CASE
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename R>
+static int zfft_dispatch(const Dims &D, int np, const void *Hp, void *P0, size_t pstride, int row0, int nrows,
+                         const Tables<R> &tb, hipStream_t s) {
+  switch (D.nzd) {
+#define CASE(n) case n: return run_zfft<R, n>(D, np, Hp, P0, pstride, row0, nrows, tb, s);
+    HST_FOR_SIZES(CASE)
+#undef CASE
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename R>
+static int xpass_dispatch(const Dims &D, int mode, const void *Hv, void *Hp, int row0, int nrows, const Tables<R> &tb,
+                          const double *dyl, double *cfl_part, int want_cfl, int *n_part, hipStream_t s) {
+  switch (D.nxd) {
+#define CASE(n) case n: return run_xpass<R, n>(D, mode, Hv, Hp, row0, nrows, tb, dyl, cfl_part, want_cfl, n_part, s);
+    HST_FOR_SIZES(CASE)
+#undef CASE
+  }
+  return hipErrorInvalidValue;
+}
+
+int launch_zift(const Dims &D, int mixed, const double2 *const V[3], void *Hv, int row0, int nrows,
+                const void *tables, hipStream_t s) {
+  const TableSet *t = (const TableSet *)tables;
+  return mixed ? zift_dispatch<float>(D, V, Hv, row0, nrows, t->f, s)
+               : zift_dispatch<double>(D, V, Hv, row0, nrows, t->d, s);
+}
+
+int launch_xpass(const Dims &D, int mixed, int mode, const void *Hv, void *Hp, int row0, int nrows,
+                 const void *tables, const double *dyl, double *cfl_part, int want_cfl, int *n_part, hipStream_t s) {
+  const TableSet *t = (const TableSet *)tables;
+  return mixed ? xpass_dispatch<float>(D, mode, Hv, Hp, row0, nrows, t->f, dyl, cfl_part, want_cfl, n_part, s)
+               : xpass_dispatch<double>(D, mode, Hv, Hp, row0, nrows, t->d, dyl, cfl_part, want_cfl, n_part, s);
+}
+
+int launch_zfft(const Dims &D, int mixed, int np, const void *Hp, void *P0, size_t pstride, int row0, int nrows,
+                const void *tables, hipStream_t s) {
+  const TableSet *t = (const TableSet *)tables;
+  return mixed ? zfft_dispatch<float>(D, np, Hp, P0, pstride, row0, nrows, t->f, s)
+               : zfft_dispatch<double>(D, np, Hp, P0, pstride, row0, nrows, t->d, s);
+}
+
+// Algorithmic bytes (each array read or written once; interior of the chunk).
+double bytes_zift(const Dims &D, int mixed, int nrows) {
+  const double cs = mixed ? 8.0 : 16.0;
+  return 3.0 * nrows * ((double)D.ncol * 16.0 + (double)D.NXm * D.nzd * cs);
+}
+double bytes_xpass(const Dims &D, int mixed, int mode, int nrows) {
+  const double cs = mixed ? 8.0 : 16.0, h = (double)D.NXm * D.nzd * nrows * cs;
+  return mode == 0 ? 9.0 * h : (mode == 1 ? 6.0 * h : 3.0 * h);
+}
+double bytes_zfft(const Dims &D, int mixed, int np, int nrows) {
+  const double cs = mixed ? 8.0 : 16.0;
+  return np * (double)nrows * ((double)D.NXm * D.nzd * cs + (double)D.ncol * cs);
+}
+
+}  // namespace hst

@@ -1,0 +1,120 @@
+// Internal declarations of libhstgpu (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hst_gpu.h"
+
+namespace hst {
+
+void set_error(const std::string &msg);
+std::string check_deck(const hst_deck &d);
+void grid_setup(const hst_deck &d, hst_grid_info &g, std::vector<double> &y, std::vector<double> &dyl,
+                std::vector<double> &der);
+void initial_field(const hst_deck &d, const std::vector<double> &y, std::complex<double> *V);
+
+// The factorisation of the D0 matrix of one line (bordering, one slab): per
+// interior row i = 0..ny-3 the multipliers of rows i-2 and i-1, the inverse
+// pivot, the upper entries, the border columns and the weights of rows 0
+// and 1 of the inverse unit upper factor; and the real half of the record.
+struct D0Factor {
+  enum { L2 = 0, L1, RP, U1, U2, B1, B2, W, V, NCOL };
+  std::vector<double> rows;  // [(ny-2) * NCOL]
+  double rec[25];            // record entries by their 1-based index; rec[0] = U1(ny-4)
+  double c1m4, c1m3, d11, d12, c10, c2m3, d21, d22, c20, c21;  // border-row coefficients
+};
+void d0_factor(const std::vector<double> &der, int ny, D0Factor &f);
+
+// ---------------------------------------------------------------- device side
+
+// Sizes and constants every kernel takes by value.
+struct Dims {
+  int nx, ny, nz, NXm, NZm, NYg, ncol, nxd, nzd, NXp;
+  double alfa0, beta0, ni, S, factor, dx, dz;
+};
+
+// Border constants of the D0 solve, by value.
+struct D0Const {
+  double rec[25];
+  double c1m4, c1m3, d11, d12, c10, c2m3, d21, d22, c20, c21;
+};
+
+// Workspace of one cyclic pentadiagonal sweep: [(ny-2) rows][ncol] each.
+struct PentaWS {
+  double *U1, *U2, *B1, *B2;
+  double2 *XR;
+};
+
+// Kernel launchers (hst_kernels.hip); all enqueue on `stream` and return
+// hipError_t as int.  Field pointers are double2 [NYg][ncol] arrays.
+struct FftTables;  // twiddles, per precision
+
+bool fft_size_supported(int n);
+int make_fft_tables(const Dims &D, int mixed, void **tables);
+void free_fft_tables(void *tables);
+
+// The transform round trip for rows [row0, row0 + nrows) of the chunk:
+// k_zift (z inverse of u, v, w into Hv), k_xpass (x inverse, the products
+// or, mode 1, the velocities times factor, or, mode 2, nothing but the CFL
+// partial maxima; x forward into Hp), k_zfft (z forward of np spectra into
+// P0 + p*pstride, the spectral layout).
+int launch_zift(const Dims &D, int mixed, const double2 *const V[3], void *Hv, int row0, int nrows,
+                const void *tables, hipStream_t s);
+int launch_xpass(const Dims &D, int mixed, int mode, const void *Hv, void *Hp, int row0, int nrows,
+                 const void *tables, const double *dyl, double *cfl_part, int want_cfl, int *n_part, hipStream_t s);
+int launch_zfft(const Dims &D, int mixed, int np, const void *Hp, void *P0, size_t pstride, int row0, int nrows,
+                const void *tables, hipStream_t s);
+double bytes_zift(const Dims &D, int mixed, int nrows);
+double bytes_xpass(const Dims &D, int mixed, int mode, int nrows);
+double bytes_zfft(const Dims &D, int mixed, int np, int nrows);
+
+// buildrhs_prepare + buildrhs(1) + buildrhs(2) (+ the default shift when shift).
+// nonlinear: 0 buildrhs_prepare only, 1 or 2 buildrhs(ODE, g) alone, 3 all.
+struct RhsArgs {
+  double ode0, ode1, ode2, deltat, s2now, dgx, dgz;
+  int nonlinear, shift;
+};
+int launch_phase4(const Dims &D, double2 *R1, double2 *R2, double2 *O1, double2 *O2, const double *y, double dgx,
+                  double dgz, hipStream_t s);
+int launch_buildrhs(const Dims &D, int mixed, const RhsArgs &a, const double2 *V1, const double2 *V2,
+                    const double2 *V3, const void *const P[6], double2 *R1, double2 *R2, double2 *O1, double2 *O2,
+                    const double *der, const double *y, hipStream_t s);
+
+// Cyclic pentadiagonal solves with per-column rows (D2V, ETA, POISSON).
+// nsys systems in one launch: system k solves kinds[k] from src[k] into dst[k].
+int launch_penta(const Dims &D, int nsys, const int *kinds, double lambda, const double2 *const *src,
+                 double2 *const *dst, const PentaWS *ws, const double *der, double sx, double sz, hipStream_t s);
+
+// D0-type solves with the shared factorisation.  rhs_mode 0: D0 x = src;
+// 1: D0 x = D1 src (src ghost rows filled).  out_mode 0: x into dst;
+// 1: u, w recovered from x = dv/dy and eta (dst = V1 holding eta, dst3 = V3);
+// 2: sum of w |x|^2 into part (one double per block).
+int launch_d0solve(const Dims &D, int rhs_mode, int out_mode, const double2 *src, double2 *dst, double2 *dst3,
+                   double2 *XR, const double *f0rows, const D0Const &fc, const double *der, const double *dyl,
+                   double sx, double sz, double *part, int *n_part, hipStream_t s);
+
+// The exact shift of q[0..nq) (rhs1, rhs2, oldrhs1, oldrhs2): D0 solve at the
+// old shifts, node phase, ghosts at the new shifts, D0 re-weight.
+int launch_exact_shift(const Dims &D, int nq, double2 *const *q, double2 *const *XR, const double *f0rows,
+                       const D0Const &fc, const double *der, const double *y, double dgx, double dgz, double sx0,
+                       double sz0, double sx1, double sz1, hipStream_t s);
+
+// Ghost rows of nf fields at the shifts (sx, sz).
+int launch_ghosts(const Dims &D, int nf, double2 *const *f, double sx, double sz, hipStream_t s);
+
+// First reduction of the statistics: per block 9 sums (uu vv ww uv vw uw eps mfx mfz).
+int launch_stats(const Dims &D, const double2 *V1, const double2 *V2, const double2 *V3, const double *dyl,
+                 double *part, int *n_part, hipStream_t s);
+
+// Layout conversion between the reference layout (host) and the device layout.
+int launch_to_device_layout(const Dims &D, const double2 *ref, double2 *dev, hipStream_t s);
+int launch_to_ref_layout(const Dims &D, const double2 *dev, double2 *ref, hipStream_t s);
+int launch_product_to_field(const Dims &D, int mixed, const void *P, double2 *dst, hipStream_t s);
+
+
+}  // namespace hst

@@ -1,0 +1,762 @@
+// Kernels of the spectral side of the RK3 substep on gfx950: the right-hand
+// sides, the shear shift, the cyclic pentadiagonal line solves, the recovery
+// of u and w, the ghost rows and the statistics.
+//
+// Layout: a field is [row][col] with row = iy + 2 (iy = -2..ny+1) and
+// col = ix*(2nz+1) + iz + nz, so one lane owns one Fourier column and a
+// wavefront reads 64 consecutive columns of a row in one coalesced access.
+// The compact-stencil weights der(iy, n, j) are the same for every lane of a
+// wavefront (all lanes are at the same row), so they come through the scalar
+// cache.  Every kernel with a recurrence in y runs one column per lane.
+#include <hip/hip_runtime.h>
+
+#include "hst_cx.h"
+#include "hst_internal.h"
+
+namespace hst {
+
+typedef Cx<double> C;
+
+enum { KD2V = 1, KETA = 2, KPOISSON = 3, KD0 = 4, KDY = 5 };
+
+__device__ __forceinline__ void col_ix_iz(const Dims &D, int col, int &ix, int &iz) {
+  ix = col / D.NZm;
+  iz = col - ix * D.NZm - D.nz;
+}
+
+__device__ __forceinline__ C cexpi(double th) {
+  double s, c;
+  sincos(th, &s, &c);
+  return C{c, s};
+}
+
+// der(iy, n, j) on the device: [iy][n][j + 2]
+__device__ __forceinline__ double DER(const double *der, int iy, int n, int j) { return der[iy * 20 + n * 5 + j + 2]; }
+
+// ------------------------------------------------------------------ right-hand sides
+
+template <typename PR>
+struct PTraits;
+template <>
+struct PTraits<double> {
+  typedef double2 T2;
+};
+template <>
+struct PTraits<float> {
+  typedef float2 T2;
+};
+
+template <typename PR>
+__device__ __forceinline__ Cx<PR> ldp(const typename PTraits<PR>::T2 *p) {
+  typename PTraits<PR>::T2 v = *p;
+  return Cx<PR>{v.x, v.y};
+}
+
+// buildrhs_prepare + buildrhs(g = 1, 2) (+ the default node-phase shift) of one
+// column over a chunk of rows, with five-row windows of u, v, w and of the six
+// product spectra in registers.  PR: the arithmetic of the product stencils.
+// MODE 3: prepare and both product groups (the fused substep); 0: prepare only;
+// 1, 2: add the products of group 1 or 2 to rhs, oldrhs as buildrhs(ODE, g).
+template <typename PR, int MODE, bool SHIFT>
+__global__ void __launch_bounds__(256)
+    k_buildrhs(Dims D, RhsArgs a, const double2 *__restrict__ V1, const double2 *__restrict__ V2,
+               const double2 *__restrict__ V3, const typename PTraits<PR>::T2 *__restrict__ P0, size_t pstride,
+               double2 *__restrict__ R1, double2 *__restrict__ R2, double2 *__restrict__ O1,
+               double2 *__restrict__ O2, const double *__restrict__ der, const double *__restrict__ y, int rch) {
+  constexpr bool NONLIN = MODE != 0, PREP = (MODE == 0 || MODE == 3);
+  constexpr int Q0 = (MODE == 2 ? 3 : 0), Q1 = (MODE == 1 ? 3 : 6);
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= D.ncol) return;
+  int ix, iz;
+  col_ix_iz(D, col, ix, iz);
+  const double al = D.alfa0 * ix, be = D.beta0 * iz;
+  const double kk = al * al + be * be;
+  const bool m00 = (ix == 0 && iz == 0);
+  const double gam = a.s2now * (ix * D.alfa0) - D.S * (iz * D.beta0);
+  const int iy0 = blockIdx.y * rch, iy1 = min(iy0 + rch, D.ny);
+  const size_t nc = D.ncol;
+  // windows: w[k] holds row iy - 2 + k
+  C u[5], v[5], w[5];
+  Cx<PR> p[6][5];
+#pragma unroll
+  for (int k = 1; k < 5; k++) {
+    const size_t r = (size_t)(iy0 - 2 + k - 1 + 2) * nc + col;
+    if constexpr (PREP) { u[k] = ld(V1 + r); v[k] = ld(V2 + r); w[k] = ld(V3 + r); }
+    if constexpr (NONLIN) {
+#pragma unroll
+      for (int q = Q0; q < Q1; q++) p[q][k] = ldp<PR>(P0 + q * pstride + r);
+    }
+  }
+  const PR alp = (PR)al, bep = (PR)be, kkp = (PR)kk;
+  for (int iy = iy0; iy < iy1; iy++) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if constexpr (PREP) { u[k] = u[k + 1]; v[k] = v[k + 1]; w[k] = w[k + 1]; }
+      if constexpr (NONLIN) {
+#pragma unroll
+        for (int q = Q0; q < Q1; q++) p[q][k] = p[q][k + 1];
+      }
+    }
+    {
+      const size_t r = (size_t)(iy + 2 + 2) * nc + col;
+      if constexpr (PREP) { u[4] = ld(V1 + r); v[4] = ld(V2 + r); w[4] = ld(V3 + r); }
+      if constexpr (NONLIN) {
+#pragma unroll
+        for (int q = Q0; q < Q1; q++) p[q][4] = ldp<PR>(P0 + q * pstride + r);
+      }
+    }
+    const size_t c0 = (size_t)(iy + 2) * nc + col;
+    C r1, r2, o1, o2;
+    if constexpr (!PREP) {
+      r1 = ld(R1 + c0); r2 = ld(R2 + c0); o1 = ld(O1 + c0); o2 = ld(O2 + c0);
+    } else {
+    // v equation (src/hst_equations.f90:108-117)
+    C unkn{0, 0}, impl{0, 0};
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+      const double d0 = DER(der, iy, 0, j - 2), d2 = DER(der, iy, 2, j - 2), d4 = DER(der, iy, 3, j - 2);
+      const double helm = d2 - kk * d0;
+      const double biharm = d4 - 2.0 * kk * d2 + kk * kk * d0;
+      unkn = unkn + helm * v[j];
+      impl = impl + (D.ni * biharm) * v[j];
+    }
+    C o2old = ld(O2 + c0), o1old = ld(O1 + c0);
+    // ODE(1) unkn / deltat + impl - ODE(3) oldrhs (the damping term is zero)
+    r2 = a.ode0 * unkn;
+    r2 = C{r2.x / a.deltat, r2.y / a.deltat} + impl - a.ode2 * o2old;
+    o2 = C{0.0, 0.0};
+    // eta equation (:118-137), the (0,0) column packs (u, w)
+    C unk1{0, 0}, imp1{0, 0}, expl{0, 0};
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+      const double d0 = DER(der, iy, 0, j - 2), d2 = DER(der, iy, 2, j - 2);
+      const double helm = d2 - kk * d0;
+      if (m00) {
+        const C f{u[j].x, w[j].x};
+        unk1 = unk1 + d0 * f;
+        imp1 = imp1 + (D.ni * d2) * f;
+      } else {
+        const C f = imul(be, u[j]) - imul(al, w[j]);
+        unk1 = unk1 + d0 * f;
+        imp1 = imp1 + (D.ni * helm) * f;
+      }
+      expl = expl + imul(gam * d0, v[j]);
+    }
+    r1 = (a.ode0 * unk1);
+    r1 = C{r1.x / a.deltat, r1.y / a.deltat} + imp1 - a.ode2 * o1old + a.ode1 * expl;
+    o1 = expl;
+    }
+    if constexpr (NONLIN) {
+      // buildrhs for the six products in the order uu vv ww uv vw uw (:169-207)
+      PR e0[5], e1[5], e2[5];
+#pragma unroll
+      for (int j = 0; j < 5; j++) {
+        e0[j] = (PR)DER(der, iy, 0, j - 2);
+        e1[j] = (PR)DER(der, iy, 1, j - 2);
+        e2[j] = (PR)DER(der, iy, 2, j - 2);
+      }
+#pragma unroll
+      for (int q = Q0; q < Q1; q++) {
+        Cx<PR> d0{0, 0}, d1{0, 0}, d2{0, 0};
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+          d0 = d0 + e0[j] * p[q][j];
+          d1 = d1 + e1[j] * p[q][j];
+          d2 = d2 + e2[j] * p[q][j];
+        }
+        Cx<PR> ru{0, 0}, rw{0, 0}, ex;
+        if (q == 0) { ru = -imul(alp, d0); ex = d1 * (-(alp * alp)); }
+        else if (q == 1) { ex = kkp * d1; }
+        else if (q == 2) { rw = -imul(bep, d0); ex = d1 * (-(bep * bep)); }
+        else if (q == 3) { ru = -d1; ex = imul(alp, d2) + imul(alp * kkp, d0); }
+        else if (q == 4) { rw = -d1; ex = imul(bep, d2) + imul(bep * kkp, d0); }
+        else { ru = -imul(bep, d0); rw = -imul(alp, d0); ex = d1 * (-((PR)2 * alp * bep)); }
+        const C exd{(double)ex.x, (double)ex.y};
+        r2 = r2 + a.ode1 * exd;
+        o2 = o2 + exd;
+        C e;
+        if (m00) e = C{(double)ru.x, (double)rw.x};
+        else {
+          const Cx<PR> ep = imul(bep, ru) - imul(alp, rw);
+          e = C{(double)ep.x, (double)ep.y};
+        }
+        r1 = r1 + a.ode1 * e;
+        o1 = o1 + e;
+      }
+    }
+    if constexpr (SHIFT) {
+      // shear_shift, default advection (src/hst_equations.f90:245-257)
+      const C f = cexpi(-((D.alfa0 * ix) * a.dgx + (D.beta0 * iz) * a.dgz) * y[iy + 2]);
+      r1 = r1 * f; r2 = r2 * f; o1 = o1 * f; o2 = o2 * f;
+    }
+    st(R1 + c0, r1); st(R2 + c0, r2); st(O1 + c0, o1); st(O2 + c0, o2);
+  }
+}
+
+int launch_buildrhs(const Dims &D, int mixed, const RhsArgs &a, const double2 *V1, const double2 *V2,
+                    const double2 *V3, const void *const P[6], double2 *R1, double2 *R2, double2 *O1, double2 *O2,
+                    const double *der, const double *y, hipStream_t s) {
+  const int rch = 32;
+  dim3 grid((D.ncol + 255) / 256, (D.ny + rch - 1) / rch);
+  const size_t pstride = (size_t)D.NYg * D.ncol;
+  const int mode = a.nonlinear;  // 0 prepare, 1/2 one group, 3 all
+  const bool sh = a.shift != 0;
+#define L(PR, MD, SH)                                                                                            \
+  hipLaunchKernelGGL((k_buildrhs<PR, MD, SH>), grid, dim3(256), 0, s, D, a, V1, V2, V3,                          \
+                     (const typename PTraits<PR>::T2 *)P[0], pstride, R1, R2, O1, O2, der, y, rch)
+#define M(PR)                                                      \
+  switch (mode) {                                                  \
+    case 0: if (sh) L(PR, 0, true); else L(PR, 0, false); break;   \
+    case 1: L(PR, 1, false); break;                                \
+    case 2: L(PR, 2, false); break;                                \
+    default: if (sh) L(PR, 3, true); else L(PR, 3, false); break;  \
+  }
+  if (mixed) { M(float) } else { M(double) }
+#undef M
+#undef L
+  return hipGetLastError();
+}
+
+// shear_shift with the default advection as a pass of its own
+// (src/hst_equations.f90:245-257): rhs, oldrhs times the node phase.
+__global__ void __launch_bounds__(256)
+    k_phase4(Dims D, double2 *R1, double2 *R2, double2 *O1, double2 *O2, const double *__restrict__ y, double dgx,
+             double dgz) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= D.ncol) return;
+  int ix, iz;
+  col_ix_iz(D, col, ix, iz);
+  const int iy = blockIdx.y;
+  const size_t o = (size_t)(iy + 2) * D.ncol + col;
+  const C f = cexpi(-((D.alfa0 * ix) * dgx + (D.beta0 * iz) * dgz) * y[iy + 2]);
+  st(R1 + o, ld(R1 + o) * f); st(R2 + o, ld(R2 + o) * f);
+  st(O1 + o, ld(O1 + o) * f); st(O2 + o, ld(O2 + o) * f);
+}
+
+int launch_phase4(const Dims &D, double2 *R1, double2 *R2, double2 *O1, double2 *O2, const double *y, double dgx,
+                  double dgz, hipStream_t s) {
+  dim3 grid((D.ncol + 255) / 256, D.ny);
+  hipLaunchKernelGGL(k_phase4, grid, dim3(256), 0, s, D, R1, R2, O1, O2, y, dgx, dgz);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ the border system
+
+// The reduced (2x2) system of a line with one slab (src/hst_linsolve.f90:383-453)
+// from the record: rr = the real record entries by 1-based index, the complex
+// ones (x0 = rec 1-2, xx1 = rec 7-8, xm2 = rec 13-14, xm1 = rec 19-20) and the
+// right-hand sides of the two border rows.  Returns the border (b1, b2).
+__device__ __forceinline__ void border_solve(const double *rr, double c1m4, double c1m3, double d11, double d12,
+                                             double c10, double c2m3, double d21, double d22, double c20, double c21,
+                                             C ph, C x0, C xx1, C xm2, C xm1, C r25, C r27, C &b1, C &b2) {
+  const C cph = conj(ph), php = ph;
+  C A11{0, 0}, A12{0, 0}, A21{0, 0}, A22{0, 0}, f;
+  b1 = r25 - c1m4 * xm2 - c1m3 * xm1 - (c10 * php) * x0;
+  A11 = A11 - cph * (c1m4 * rr[15] + c1m3 * rr[21]);
+  A12 = A12 - cph * (c1m4 * rr[16] + c1m3 * rr[22]);
+  A11 = A11 + C{d11, 0.0} - C{c1m4 * rr[17] + c1m3 * rr[23], 0.0};
+  A12 = A12 + C{d12, 0.0} - C{c1m4 * rr[18] + c1m3 * rr[24], 0.0};
+  f = (c10 * php) * conj(ph);
+  A11 = A11 - f * rr[3];
+  A12 = A12 - f * rr[4];
+  A11 = A11 - (c10 * php) * rr[5];
+  A12 = A12 - (c10 * php) * rr[6];
+  b2 = r27 - c2m3 * xm1 - php * (c20 * x0 + c21 * xx1);
+  A21 = A21 - (cph * c2m3) * rr[21];
+  A22 = A22 - (cph * c2m3) * rr[22];
+  A21 = A21 + C{d21, 0.0} - C{c2m3 * rr[23], 0.0};
+  A22 = A22 + C{d22, 0.0} - C{c2m3 * rr[24], 0.0};
+  f = php * conj(ph);
+  A21 = A21 - f * (c20 * rr[3] + c21 * rr[9]);
+  A22 = A22 - f * (c20 * rr[4] + c21 * rr[10]);
+  A21 = A21 - php * (c20 * rr[5] + c21 * rr[11]);
+  A22 = A22 - php * (c20 * rr[6] + c21 * rr[12]);
+  const C piv = cdiv(C{1.0, 0.0}, A11);
+  f = A21 * piv;
+  A22 = A22 - f * A12;
+  b2 = b2 - f * b1;
+  b2 = cdiv(b2, A22);
+  b1 = cdiv(b1 - A12 * b2, A11);
+}
+
+// ------------------------------------------------------------------ per-column solves
+
+__device__ __forceinline__ double coef(int kind, double lam, double ni, double kk, const double *der, int iy, int j) {
+  const double d0 = DER(der, iy, 0, j);
+  if (kind == KD2V) {
+    const double d2 = DER(der, iy, 2, j), d4 = DER(der, iy, 3, j);
+    return lam * (d2 - kk * d0) - ni * (d4 - 2.0 * kk * d2 + kk * kk * d0);
+  }
+  if (kind == KETA) return lam * d0 - ni * (DER(der, iy, 2, j) - kk * d0);
+  if (kind == KPOISSON) return DER(der, iy, 2, j) - kk * d0;
+  return d0;
+}
+
+struct PentaArgs {
+  int kind[2];
+  const double2 *src[2];
+  double2 *dst[2];
+  PentaWS ws[2];
+};
+
+// One cyclic pentadiagonal line per lane (line_solve, src/hst_linsolve.f90:147-483,
+// one slab): forward elimination of rows 0..ny-3 with the rows generated from
+// der, record in registers, 2x2 border system, back substitution.
+__global__ void __launch_bounds__(256)
+    k_penta(Dims D, PentaArgs pa, double lam, const double *__restrict__ der, double sx, double sz) {
+  const int sys = blockIdx.y;
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= D.ncol) return;
+  const int kind = pa.kind[sys];
+  const double2 *__restrict__ src = pa.src[sys];
+  double2 *__restrict__ dst = pa.dst[sys];
+  const PentaWS ws = pa.ws[sys];
+  int ix, iz;
+  col_ix_iz(D, col, ix, iz);
+  const size_t nc = D.ncol;
+  const int m = D.ny, mi = m - 2;
+  if (ix == 0 && iz == 0 && kind == KD2V) {
+    for (int i = 0; i < m; i++) st(dst + (size_t)(i + 2) * nc + col, C{0.0, 0.0});
+    return;
+  }
+  if (ix == 0 && iz == 0 && kind == KPOISSON) return;
+  const double al = D.alfa0 * ix, be = D.beta0 * iz;
+  const double kk = al * al + be * be, ni = D.ni;
+  double u11 = 0, u12 = 0, p1 = 0, q1 = 0, u21 = 0, u22 = 0, p2 = 0, q2 = 0;
+  double w1 = 0, w2 = 0, v1 = 0, v2 = 0, sp0 = 0, sq0 = 0, sp1 = 0, sq1 = 0;
+  C x1{0, 0}, x2{0, 0}, sx0{0, 0}, sx1{0, 0};
+  for (int i = 0; i < mi; i++) {
+    double a[5];
+#pragma unroll
+    for (int j = 0; j < 5; j++) a[j] = coef(kind, lam, ni, kk, der, i, j - 2);
+    double p = 0.0, q = 0.0;
+    if (i == 0) { p = a[0]; q = a[1]; }
+    if (i == 1) q = a[0];
+    C bx = ld(src + (size_t)(i + 2) * nc + col);
+    if (i >= 2) {
+      const double l = a[0];
+      a[1] = a[1] - l * u21;
+      a[2] = a[2] - l * u22;
+      bx = bx - l * x2; p = p - l * p2; q = q - l * q2;
+    }
+    if (i >= 1) {
+      const double l = a[1];
+      a[2] = a[2] - l * u11;
+      a[3] = a[3] - l * u12;
+      bx = bx - l * x1; p = p - l * p1; q = q - l * q1;
+    }
+    const double rp = 1.0 / a[2];
+    const double t1 = a[3] * rp, t2 = a[4] * rp;
+    bx = bx * rp; p = p * rp; q = q * rp;
+    const size_t o = (size_t)i * nc + col;
+    ws.U1[o] = t1; ws.U2[o] = t2; ws.B1[o] = p; ws.B2[o] = q; st(ws.XR + o, bx);
+    double ww = -(u11 * w1 + u22 * w2);
+    if (i == 0) ww = 1.0;
+    double vv = -(u11 * v1 + u22 * v2);
+    if (i == 1) vv = 1.0;
+    sx0 = sx0 + ww * bx; sp0 = sp0 + ww * p; sq0 = sq0 + ww * q;
+    sx1 = sx1 + vv * bx; sp1 = sp1 + vv * p; sq1 = sq1 + vv * q;
+    w2 = w1; w1 = ww; v2 = v1; v1 = vv;
+    u21 = u11; u22 = u12; x2 = x1; p2 = p1; q2 = q1;
+    u11 = t1; u12 = t2; x1 = bx; p1 = p; q1 = q;
+  }
+  double rr[25];
+  rr[3] = sp0; rr[4] = sq0; rr[5] = w2 * u22 + w1 * u11; rr[6] = w1 * u12;
+  rr[9] = sp1; rr[10] = sq1; rr[11] = v2 * u22 + v1 * u11; rr[12] = v1 * u12;
+  rr[15] = p2 - u21 * p1; rr[16] = q2 - u21 * q1; rr[17] = u22 - u21 * u11; rr[18] = -u21 * u12;
+  rr[21] = p1; rr[22] = q1; rr[23] = u11; rr[24] = u12;
+  const C xm2 = x2 - u21 * x1, xm1 = x1;
+  const C r25 = ld(src + (size_t)(m - 2 + 2) * nc + col), r27 = ld(src + (size_t)(m - 1 + 2) * nc + col);
+  const int ga = m - 2;
+  const C ph = cexpi(-(D.alfa0 * ix * sx + D.beta0 * iz * sz));
+  C b1, b2;
+  border_solve(rr, coef(kind, lam, ni, kk, der, ga, -2), coef(kind, lam, ni, kk, der, ga, -1),
+               coef(kind, lam, ni, kk, der, ga, 0), coef(kind, lam, ni, kk, der, ga, 1),
+               coef(kind, lam, ni, kk, der, ga, 2), coef(kind, lam, ni, kk, der, ga + 1, -2),
+               coef(kind, lam, ni, kk, der, ga + 1, -1), coef(kind, lam, ni, kk, der, ga + 1, 0),
+               coef(kind, lam, ni, kk, der, ga + 1, 1), coef(kind, lam, ni, kk, der, ga + 1, 2), ph, sx0, sx1, xm2,
+               xm1, r25, r27, b1, b2);
+  st(dst + (size_t)(m - 2 + 2) * nc + col, b1);
+  st(dst + (size_t)(m - 1 + 2) * nc + col, b2);
+  const C cph = conj(ph);
+  const C yb1 = cph * b1, yb2 = cph * b2;
+  C xk1 = b1, xk2 = b2;
+#pragma unroll 4
+  for (int i = mi - 1; i >= 0; i--) {
+    const size_t o = (size_t)i * nc + col;
+    const C xk = ld(ws.XR + o) - ws.B1[o] * yb1 - ws.B2[o] * yb2 - ws.U1[o] * xk1 - ws.U2[o] * xk2;
+    st(dst + (size_t)(i + 2) * nc + col, xk);
+    xk2 = xk1;
+    xk1 = xk;
+  }
+}
+
+int launch_penta(const Dims &D, int nsys, const int *kinds, double lambda, const double2 *const *src,
+                 double2 *const *dst, const PentaWS *ws, const double *der, double sx, double sz, hipStream_t s) {
+  PentaArgs pa;
+  for (int k = 0; k < nsys; k++) {
+    pa.kind[k] = kinds[k];
+    pa.src[k] = src[k];
+    pa.dst[k] = dst[k];
+    pa.ws[k] = ws[k];
+  }
+  dim3 grid((D.ncol + 255) / 256, nsys);
+  hipLaunchKernelGGL(k_penta, grid, dim3(256), 0, s, D, pa, lambda, der, sx, sz);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ shared-factor D0 solves
+
+// rows table: [i][L2 L1 RP U1 U2 B1 B2 W V]
+#define F0(i, k) f0[(i) * 9 + (k)]
+
+// Forward sweep of D0 x = r for one lane: r(i) from the functor, XR stored.
+template <class RHS>
+__device__ __forceinline__ void d0_forward(const Dims &D, const double *__restrict__ f0, RHS rhs, double2 *XR, int col,
+                                           C &x1, C &x2, C &sx0, C &sx1) {
+  const size_t nc = D.ncol;
+  const int mi = D.ny - 2;
+  x1 = C{0, 0}; x2 = C{0, 0}; sx0 = C{0, 0}; sx1 = C{0, 0};
+  for (int i = 0; i < mi; i++) {
+    C bx = rhs(i);
+    if (i >= 2) bx = bx - F0(i, 0) * x2;
+    if (i >= 1) bx = bx - F0(i, 1) * x1;
+    bx = bx * F0(i, 2);
+    st(XR + (size_t)i * nc + col, bx);
+    sx0 = sx0 + F0(i, 7) * bx;
+    sx1 = sx1 + F0(i, 8) * bx;
+    x2 = x1;
+    x1 = bx;
+  }
+}
+
+__device__ __forceinline__ void d0_border(const D0Const &fc, C ph, C x1, C x2, C sx0, C sx1, C r25, C r27, C &b1,
+                                          C &b2) {
+  const C xm2 = x2 - fc.rec[0] * x1, xm1 = x1;
+  border_solve(fc.rec, fc.c1m4, fc.c1m3, fc.d11, fc.d12, fc.c10, fc.c2m3, fc.d21, fc.d22, fc.c20, fc.c21, ph, sx0,
+               sx1, xm2, xm1, r25, r27, b1, b2);
+}
+
+// D0 x = src (RHS 0) or D0 x = D1 src (RHS 1); OUT 0: x to dst; 1: recover
+// u, w from x = dv/dy and eta = dst (src/hst_equations.f90:303-324); 2: block
+// sums of w |x|^2 (the dissipation's y derivatives, src/hst_stats.f90:200-223).
+template <int RHS, int OUT>
+__global__ void __launch_bounds__(256)
+    k_d0solve(Dims D, D0Const fc, const double *__restrict__ f0, const double *__restrict__ der,
+              const double *__restrict__ dyl, const double2 *__restrict__ src, double2 *__restrict__ dst,
+              double2 *__restrict__ dst3, double2 *__restrict__ XR, double sx, double sz, double *__restrict__ part) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = col < D.ncol;
+  const int cl = act ? col : 0;
+  int ix, iz;
+  col_ix_iz(D, cl, ix, iz);
+  const size_t nc = D.ncol;
+  const int m = D.ny;
+  double acc = 0.0;
+  if (act) {
+    auto rhs = [&](int i) -> C {
+      if constexpr (RHS == 0) {
+        return ld(src + (size_t)(i + 2) * nc + col);
+      } else {
+        C r{0, 0};
+#pragma unroll
+        for (int j = -2; j <= 2; j++) r = r + DER(der, i, 1, j) * ld(src + (size_t)(i + j + 2) * nc + col);
+        return r;
+      }
+    };
+    C x1, x2, sx0, sx1;
+    d0_forward(D, f0, rhs, XR, col, x1, x2, sx0, sx1);
+    const C r25 = rhs(m - 2), r27 = rhs(m - 1);
+    const C ph = cexpi(-(D.alfa0 * ix * sx + D.beta0 * iz * sz));
+    C b1, b2;
+    d0_border(fc, ph, x1, x2, sx0, sx1, r25, r27, b1, b2);
+    const C cph = conj(ph);
+    const C yb1 = cph * b1, yb2 = cph * b2;
+    const double al = D.alfa0 * ix, be = D.beta0 * iz, kk = al * al + be * be;
+    const bool m00 = (ix == 0 && iz == 0);
+    auto emit = [&](int i, C x) {
+      const size_t o = (size_t)(i + 2) * nc + col;
+      if constexpr (OUT == 0) {
+        st(dst + o, x);
+      } else if constexpr (OUT == 1) {
+        const C eta = ld(dst + o);
+        if (m00) {
+          st(dst + o, C{eta.x, 0.0});
+          st(dst3 + o, C{eta.y, 0.0});
+        } else {
+          const C t = imul(al, x) - imul(be, eta);
+          const C ww = imul(be, x) + imul(al, eta);
+          st(dst + o, C{t.x / kk, t.y / kk});
+          st(dst3 + o, C{ww.x / kk, ww.y / kk});
+        }
+      } else {
+        const double wgt = (ix == 0 ? dyl[i] : 2.0 * dyl[i]);
+        acc += wgt * (x.x * x.x + x.y * x.y);
+      }
+    };
+    emit(m - 2, b1);
+    emit(m - 1, b2);
+    C xk1 = b1, xk2 = b2;
+#pragma unroll 4
+    for (int i = m - 3; i >= 0; i--) {
+      const C xk = ld(XR + (size_t)i * nc + col) - F0(i, 5) * yb1 - F0(i, 6) * yb2 - F0(i, 3) * xk1 - F0(i, 4) * xk2;
+      emit(i, xk);
+      xk2 = xk1;
+      xk1 = xk;
+    }
+  }
+  if constexpr (OUT == 2) {
+    __shared__ double red[4];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  }
+}
+
+int launch_d0solve(const Dims &D, int rhs_mode, int out_mode, const double2 *src, double2 *dst, double2 *dst3,
+                   double2 *XR, const double *f0rows, const D0Const &fc, const double *der, const double *dyl,
+                   double sx, double sz, double *part, int *n_part, hipStream_t s) {
+  dim3 grid((D.ncol + 255) / 256);
+  if (n_part) *n_part = grid.x;
+#define L(R, O) \
+  hipLaunchKernelGGL((k_d0solve<R, O>), grid, dim3(256), 0, s, D, fc, f0rows, der, dyl, src, dst, dst3, XR, sx, sz, part)
+  if (rhs_mode == 0) {
+    if (out_mode == 0) L(0, 0); else if (out_mode == 1) L(0, 1); else L(0, 2);
+  } else {
+    if (out_mode == 0) L(1, 0); else if (out_mode == 1) L(1, 1); else L(1, 2);
+  }
+#undef L
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ exact shift
+
+// shift_unweighted (src/hst_equations.f90:263-293) of one array per lane:
+// x = D0^-1 q at the old shifts, x' = x exp(-i (alfa dgx + beta dgz) y),
+// ghosts of x' at the new shifts, q = D0 x'.  The back substitution yields
+// x from the top row down, rows 0 and 1 of x come from the record, so the
+// re-weighted rows are emitted in the same sweep.
+struct QList {
+  double2 *q[4];
+  double2 *xr[4];
+};
+
+__global__ void __launch_bounds__(256)
+    k_exact_shift(Dims D, D0Const fc, QList ql, const double *__restrict__ f0, const double *__restrict__ der,
+                  const double *__restrict__ y, double dgx, double dgz, double sx0s, double sz0s, double sx1s,
+                  double sz1s) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= D.ncol) return;
+  double2 *__restrict__ q = ql.q[blockIdx.y];
+  double2 *__restrict__ XR = ql.xr[blockIdx.y];
+  int ix, iz;
+  col_ix_iz(D, col, ix, iz);
+  const size_t nc = D.ncol;
+  const int m = D.ny;
+  auto rhs = [&](int i) -> C { return ld(q + (size_t)(i + 2) * nc + col); };
+  C x1, x2, sx0, sx1;
+  d0_forward(D, f0, rhs, XR, col, x1, x2, sx0, sx1);
+  const C r25 = rhs(m - 2), r27 = rhs(m - 1);
+  const C ph0 = cexpi(-(D.alfa0 * ix * sx0s + D.beta0 * iz * sz0s));
+  C b1, b2;
+  d0_border(fc, ph0, x1, x2, sx0, sx1, r25, r27, b1, b2);
+  const C cph = conj(ph0);
+  const C yb1 = cph * b1, yb2 = cph * b2;
+  // rows 0 and 1 from the record: x = value - cph (p b1 + q b2) - (t b1 + u b2)
+  const double *rr = fc.rec;
+  const C xr0 = sx0 - cph * (rr[3] * b1 + rr[4] * b2) - (rr[5] * b1 + rr[6] * b2);
+  const C xr1 = sx1 - cph * (rr[9] * b1 + rr[10] * b2) - (rr[11] * b1 + rr[12] * b2);
+  const double th = -((D.alfa0 * ix) * dgx + (D.beta0 * iz) * dgz);
+  const C ph1 = cexpi(-(D.alfa0 * ix * sx1s + D.beta0 * iz * sz1s));
+  const C xp0 = xr0 * cexpi(th * y[2]), xp1 = xr1 * cexpi(th * y[3]);
+  const C xpm2 = b1 * cexpi(th * y[m - 2 + 2]), xpm1 = b2 * cexpi(th * y[m - 1 + 2]);
+  // window W[k] = x'(i + 1 + k)
+  C W0 = xpm2, W1 = xpm1, W2 = xp0 * ph1, W3 = xp1 * ph1;
+  C xk1 = b1, xk2 = b2;
+  for (int i = m - 3; i >= 0; i--) {
+    const C xk = ld(XR + (size_t)i * nc + col) - F0(i, 5) * yb1 - F0(i, 6) * yb2 - F0(i, 3) * xk1 - F0(i, 4) * xk2;
+    xk2 = xk1;
+    xk1 = xk;
+    const C xp = xk * cexpi(th * y[i + 2]);
+    const int r = i + 2;
+    C acc{0, 0};
+    acc = acc + DER(der, r, 0, -2) * xp;
+    acc = acc + DER(der, r, 0, -1) * W0;
+    acc = acc + DER(der, r, 0, 0) * W1;
+    acc = acc + DER(der, r, 0, 1) * W2;
+    acc = acc + DER(der, r, 0, 2) * W3;
+    st(q + (size_t)(r + 2) * nc + col, acc);
+    W3 = W2; W2 = W1; W1 = W0; W0 = xp;
+  }
+  // W = x'(0..3); ghosts below: x'(-1) = x'(ny-1) conj(ph1), x'(-2) = x'(ny-2) conj(ph1)
+  const C g1 = xpm1 * conj(ph1), g2 = xpm2 * conj(ph1);
+  C acc{0, 0};
+  acc = acc + DER(der, 1, 0, -2) * g1;
+  acc = acc + DER(der, 1, 0, -1) * W0;
+  acc = acc + DER(der, 1, 0, 0) * W1;
+  acc = acc + DER(der, 1, 0, 1) * W2;
+  acc = acc + DER(der, 1, 0, 2) * W3;
+  st(q + (size_t)(1 + 2) * nc + col, acc);
+  acc = C{0, 0};
+  acc = acc + DER(der, 0, 0, -2) * g2;
+  acc = acc + DER(der, 0, 0, -1) * g1;
+  acc = acc + DER(der, 0, 0, 0) * W0;
+  acc = acc + DER(der, 0, 0, 1) * W1;
+  acc = acc + DER(der, 0, 0, 2) * W2;
+  st(q + (size_t)(0 + 2) * nc + col, acc);
+}
+
+int launch_exact_shift(const Dims &D, int nq, double2 *const *q, double2 *const *XR, const double *f0rows,
+                       const D0Const &fc, const double *der, const double *y, double dgx, double dgz, double sx0,
+                       double sz0, double sx1, double sz1, hipStream_t s) {
+  QList ql;
+  for (int k = 0; k < nq; k++) { ql.q[k] = q[k]; ql.xr[k] = XR[k]; }
+  dim3 grid((D.ncol + 255) / 256, nq);
+  hipLaunchKernelGGL(k_exact_shift, grid, dim3(256), 0, s, D, fc, ql, f0rows, der, y, dgx, dgz, sx0, sz0, sx1, sz1);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ ghosts
+
+struct FList {
+  double2 *f[8];
+};
+
+// wrap_rows (src/hst_mpi.f90:340-355): f(ny) = f(0) ph, f(ny+1) = f(1) ph,
+// f(-1) = f(ny-1) conj(ph), f(-2) = f(ny-2) conj(ph).
+__global__ void __launch_bounds__(256) k_ghosts(Dims D, FList fl, double sx, double sz) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= D.ncol) return;
+  double2 *f = fl.f[blockIdx.y];
+  int ix, iz;
+  col_ix_iz(D, col, ix, iz);
+  const C ph = cexpi(-(D.alfa0 * ix * sx + D.beta0 * iz * sz));
+  const size_t nc = D.ncol;
+  const int ny = D.ny;
+  const C a0 = ld(f + (size_t)2 * nc + col), a1 = ld(f + (size_t)3 * nc + col);
+  const C bm1 = ld(f + (size_t)(ny + 1) * nc + col), bm2 = ld(f + (size_t)ny * nc + col);
+  st(f + (size_t)(ny + 2) * nc + col, a0 * ph);
+  st(f + (size_t)(ny + 3) * nc + col, a1 * ph);
+  st(f + (size_t)1 * nc + col, bm1 * conj(ph));
+  st(f + (size_t)0 * nc + col, bm2 * conj(ph));
+}
+
+int launch_ghosts(const Dims &D, int nf, double2 *const *f, double sx, double sz, hipStream_t s) {
+  FList fl;
+  for (int k = 0; k < nf; k++) fl.f[k] = f[k];
+  dim3 grid((D.ncol + 255) / 256, nf);
+  hipLaunchKernelGGL(k_ghosts, grid, dim3(256), 0, s, D, fl, sx, sz);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ statistics
+
+// The first reduction of outstats (src/hst_stats.f90:153-198) without the
+// Stokes layer and damping: per block the sums uu vv ww uv vw uw eps mfx mfz.
+__global__ void __launch_bounds__(256)
+    k_stats(Dims D, const double2 *__restrict__ V1, const double2 *__restrict__ V2, const double2 *__restrict__ V3,
+            const double *__restrict__ dyl, double *__restrict__ part) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (col < D.ncol) {
+    int ix, iz;
+    col_ix_iz(D, col, ix, iz);
+    const double al = D.alfa0 * ix, be = D.beta0 * iz, kk = al * al + be * be;
+    const size_t nc = D.ncol;
+    for (int iy = 0; iy < D.ny; iy++) {
+      const size_t o = (size_t)(iy + 2) * nc + col;
+      const C cu = ld(V1 + o), cv = ld(V2 + o), cw = ld(V3 + o);
+      const double dy = dyl[iy];
+      if (ix == 0 && iz == 0) {
+        s[7] += cu.x * dy;
+        s[8] += cw.x * dy;
+        s[0] += dy * (cu.x * cu.x + cu.y * cu.y);
+        s[2] += dy * (cw.x * cw.x + cw.y * cw.y);
+        s[5] += dy * (cu.x * cw.x + cu.y * cw.y);
+        continue;
+      }
+      const double w = ix == 0 ? dy : 2.0 * dy;
+      const double uu = cu.x * cu.x + cu.y * cu.y, vv = cv.x * cv.x + cv.y * cv.y, wv = cw.x * cw.x + cw.y * cw.y;
+      s[0] += w * uu;
+      s[1] += w * vv;
+      s[2] += w * wv;
+      s[3] += w * (cu.x * cv.x + cu.y * cv.y);
+      s[4] += w * (cw.x * cv.x + cw.y * cv.y);
+      s[5] += w * (cu.x * cw.x + cu.y * cw.y);
+      s[6] += w * kk * (uu + vv + wv);
+    }
+  }
+  __shared__ double red[4][9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    double v = s[k];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 9) {
+    const int k = threadIdx.x;
+    part[(size_t)blockIdx.x * 9 + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+  }
+}
+
+int launch_stats(const Dims &D, const double2 *V1, const double2 *V2, const double2 *V3, const double *dyl,
+                 double *part, int *n_part, hipStream_t s) {
+  dim3 grid((D.ncol + 255) / 256);
+  if (n_part) *n_part = grid.x;
+  hipLaunchKernelGGL(k_stats, grid, dim3(256), 0, s, D, V1, V2, V3, dyl, part);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ layouts
+
+// reference layout (col-major f(iy, iz, ix): row fastest) <-> device [row][col]
+__global__ void k_transpose_c(const double2 *__restrict__ A, double2 *__restrict__ B, int n1, int n2) {
+  // B[j][i] = A[i][j], A is n1 x n2 (row-major, j fastest)
+  __shared__ double2 t[32][33];
+  const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
+  for (int k = threadIdx.y; k < 32; k += 8) {
+    const int i = i0 + k, j = j0 + threadIdx.x;
+    if (i < n1 && j < n2) t[k][threadIdx.x] = A[(size_t)i * n2 + j];
+  }
+  __syncthreads();
+  for (int k = threadIdx.y; k < 32; k += 8) {
+    const int j = j0 + k, i = i0 + threadIdx.x;
+    if (i < n1 && j < n2) B[(size_t)j * n1 + i] = t[threadIdx.x][k];
+  }
+}
+
+int launch_to_device_layout(const Dims &D, const double2 *ref, double2 *dev, hipStream_t s) {
+  // ref: [col][row] (row fastest)  ->  dev: [row][col]
+  dim3 grid((D.NYg + 31) / 32, (D.ncol + 31) / 32);
+  hipLaunchKernelGGL(k_transpose_c, grid, dim3(32, 8), 0, s, ref, dev, D.ncol, D.NYg);
+  return hipGetLastError();
+}
+
+int launch_to_ref_layout(const Dims &D, const double2 *dev, double2 *ref, hipStream_t s) {
+  dim3 grid((D.ncol + 31) / 32, (D.NYg + 31) / 32);
+  hipLaunchKernelGGL(k_transpose_c, grid, dim3(32, 8), 0, s, dev, ref, D.NYg, D.ncol);
+  return hipGetLastError();
+}
+
+template <typename PR>
+__global__ void k_product_to_field(const typename PTraits<PR>::T2 *__restrict__ P, double2 *__restrict__ dst, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const Cx<PR> v = ldp<PR>(P + i);
+    dst[i] = make_double2((double)v.x, (double)v.y);
+  }
+}
+
+int launch_product_to_field(const Dims &D, int mixed, const void *P, double2 *dst, hipStream_t s) {
+  const size_t n = (size_t)D.NYg * D.ncol;
+  dim3 grid((unsigned)((n + 255) / 256));
+  if (mixed)
+    hipLaunchKernelGGL(k_product_to_field<float>, grid, dim3(256), 0, s, (const float2 *)P, dst, n);
+  else
+    hipLaunchKernelGGL(k_product_to_field<double>, grid, dim3(256), 0, s, (const double2 *)P, dst, n);
+  return hipGetLastError();
+}
+
+}  // namespace hst

@@ -81,6 +81,12 @@ int hst_grid_setup(const hst_deck *deck, hst_grid_info *info, double *y, double 
  * (src/hst_initial.f90:27-67, 76-108). */
 int hst_initial_field(const hst_deck *deck, double *V_host /* complex128[3][nx+1][2nz+1][ny+4] */);
 
+/* <u_i u_i> of the seeded random field of the deck on a uniform y grid,
+ * sum over the modes of w |u|^2 (w = 1 at ix = 0, else 2), which the discrete
+ * orthogonality of the y modes |m| <= ny/3 makes exact; the amplitude that
+ * gives <u_i u_i> = q2 is amplitude * sqrt(q2 / result). */
+int hst_initial_energy(const hst_deck *deck, double *q2);
+
 /* ------------------------------------------------------------------ device state */
 
 /* allocate_fields + init_fft + init_linsolve + setup_derivatives
@@ -140,10 +146,17 @@ int hst_gpu_linsolve(double lambda, double time);
  * reference does; *cfl receives the substep-3 CFL number (may be NULL). */
 int hst_gpu_timestep(double deltat, double *time, double *cfl);
 
-/* outstats' sums (src/hst_stats.f90:136-246), npy = 1, no Stokes layer or
- * damping: out[0..8] = energy, diss, uv, vw, uu, vv, ww, uw (box means,
- * as written to Runtimedata / variances_runtime.dat) and mfx, mfz. */
-int hst_gpu_outstats(double out[10]);
+/* outstats (src/hst_stats.f90:136-270) at `time` (the wrap phase of its dv/dy
+ * solves), npy = 1, no Stokes layer or damping: out[0..9] = energy, diss, uv,
+ * vw, uu, vv, ww, uw (box means, as written to Runtimedata and
+ * variances_runtime.dat), mfx, mfz. */
+int hst_gpu_outstats(double time, double out[10]);
+
+/* Rows per transform chunk (0: as many as memory allows; the physical-space
+ * buffers hold one chunk of y rows).  Takes effect at the next hst_gpu_init. */
+int hst_gpu_set_chunk_rows(int rows);
+/* Device bytes held by the library and the rows per transform chunk in use. */
+int hst_gpu_memory(int64_t *bytes, int *chunk_rows);
 
 /* ------------------------------------------------------------------ measurement */
 

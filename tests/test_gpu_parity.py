@@ -1,0 +1,314 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle, on the same
+inputs, piece by piece and for whole steps; and the reference's own golden
+fields and closed-form tests run on the GPU.
+
+FP64 tolerances: 1e-12 relative for single pieces (rounding of one pass),
+1e-10 for whole runs (the reference's regression bar).  Mixed precision
+(FP32 transforms, products and derivative right-hand sides): 1e-5.
+"""
+import dataclasses
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, deck_path
+from homogenenousshearturbulence_amd import cplio, hst
+from homogenenousshearturbulence_amd.deck import read_input
+import closed_forms
+import refops
+
+pytestmark = pytest.mark.gpu
+
+RK = hst.RK_RAI
+
+
+@pytest.fixture(scope="module")
+def O(oracle_built):
+    return oracle_built
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+class GPU:
+    """A Solver that frees the device state at the end of a with-block."""
+
+    def __init__(self, deck, precision=hst.FP64, chunk_rows=0):
+        self.s = hst.Solver(deck, precision, chunk_rows)
+
+    def __enter__(self):
+        return self.s
+
+    def __exit__(self, *a):
+        self.s.close()
+
+
+def seeded(O, d, t=0.0):
+    """Oracle and GPU states holding the same seeded field with ghosts at t."""
+    o = O.Oracle(d)
+    o.initial_field()
+    o.fill_all_ghosts(t)
+    o.time = t
+    return o
+
+
+def test_ghosts(O):
+    d = read_input(deck_path("small"))
+    F = refops.random_field(d, uniform=O.uniform_from_key)
+    with GPU(d) as g:
+        sx, sz = g.shear_shifts(0.37)
+        g.upload(hst.SCRATCH, F)
+        g.fill_ghosts(hst.SCRATCH, sx, sz)
+        G = g.download(hst.SCRATCH)
+    refops.fill_ghosts(d, F, sx, sz)
+    assert rel(G, F) <= 1e-15
+
+
+@pytest.mark.parametrize("kind", [1, 2, 3, 4, 5])
+def test_line_solve(O, kind):
+    """line_solve of every kind vs the oracle, nonzero wrap phase (t = 0.37)."""
+    d = read_input(deck_path("small"))
+    o = O.Oracle(d)
+    sx, sz = o.shear_shifts(0.37)
+    F = refops.random_field(d, uniform=O.uniform_from_key)
+    refops.fill_ghosts(d, F, sx, sz)
+    o.field(8)[...] = F
+    o.line_solve(kind, 123.4, 8, 5, sx, sz)
+    with GPU(d) as g:
+        g.upload(hst.SCRATCH, F)
+        g.upload(hst.RHS2, np.zeros_like(F))
+        g.line_solve(kind, 123.4, hst.SCRATCH, hst.RHS2, sx, sz)
+        X = g.download(hst.RHS2)
+    ref = o.field(5)
+    assert rel(X[..., 2:-2], ref[..., 2:-2]) <= 1e-12
+
+
+@pytest.mark.parametrize("precision,tol", [(hst.FP64, 1e-12), (hst.MIXED, 2e-6)])
+def test_roundtrip(O, precision, tol):
+    """test_roundtrip (tests/test_roundtrip.f90:52-77) through the GPU transforms."""
+    d = read_input(deck_path("small"))
+    V = hst.initial_field(d)
+    with GPU(d, precision) as g:
+        g.set_V(V)
+        g.fill_all_ghosts(0.0)
+        V0 = g.V
+        g.roundtrip()
+        P = np.stack([g.product(p) for p in (1, 2, 3)])
+    assert np.abs(P - V0).max() <= tol * np.abs(V0).max()
+
+
+@pytest.mark.parametrize("name", ["small", "pressure"])
+@pytest.mark.parametrize("precision,tol", [(hst.FP64, 1e-12), (hst.MIXED, 2e-5)])
+def test_products(O, name, precision, tol):
+    """The six product spectra of the transform round trip vs the oracle,
+    ghost rows included (the stencils read them)."""
+    d = read_input(deck_path(name))
+    o = seeded(O, d, 0.21)
+    o.nonlinear(mode=0)
+    P0 = o.products()
+    with GPU(d, precision) as g:
+        g.set_V(o.V)
+        g.nonlinear()
+        for p in range(6):
+            Pg = g.product(p + 1)
+            assert rel(Pg, P0[p]) <= tol, p
+
+
+@pytest.mark.parametrize("exact", [False, True])
+@pytest.mark.parametrize("s2", [False, True])
+def test_rhs_and_shift(O, exact, s2):
+    """buildrhs_prepare, buildrhs(1), buildrhs(2) and shear_shift vs the oracle."""
+    d = read_input(deck_path("small"))
+    d = dataclasses.replace(d, exact_shift=exact)
+    if s2:
+        d = dataclasses.replace(d, s2_amplitude=0.6, s2_period=0.5)
+    t = 0.13
+    o = seeded(O, d, t)
+    rng = np.random.default_rng(5)
+    for f in (6, 7):
+        o.field(f)[...] = rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape)
+    old = [o.field(6).copy(), o.field(7).copy()]
+    V0 = o.V.copy()  # (the oracle's exact shift uses V(:, :, :, 1) as scratch)
+    o.nonlinear(mode=0)
+    ode = RK[1]
+    dt = 0.002
+    o_lib = O.lib()
+    import ctypes
+    arr = (ctypes.c_double * 3)(*ode)
+    o_lib.hsto_buildrhs_prepare(o.h, arr, dt, t)
+    o_lib.hsto_buildrhs(o.h, arr, 1)
+    o_lib.hsto_buildrhs(o.h, arr, 2)
+    pre = [o.field(f).copy() for f in (4, 5, 6, 7)]
+    dt_sub = 2.0 / ode[0] * dt
+    o_lib.hsto_shear_shift(o.h, dt_sub, t)
+    post = [o.field(f).copy() for f in (4, 5, 6, 7)]
+    with GPU(d) as g:
+        g.set_V(V0)
+        g.upload(hst.OLD1, old[0])
+        g.upload(hst.OLD2, old[1])
+        g.nonlinear()
+        g.buildrhs_prepare(ode, dt, t)
+        g.buildrhs(ode, 1)
+        g.buildrhs(ode, 2)
+        got = [g.download(f) for f in (4, 5, 6, 7)]
+        for a, b in zip(got, pre):
+            assert rel(a[..., 2:-2], b[..., 2:-2]) <= 1e-12
+        g.shear_shift(dt_sub, t)
+        got = [g.download(f) for f in (4, 5, 6, 7)]
+        for a, b in zip(got, post):
+            assert rel(a[..., 2:-2], b[..., 2:-2]) <= 1e-12
+
+
+def test_linsolve(O):
+    """linsolve (src/hst_equations.f90:298-326): solves, ghosts, u and w."""
+    d = read_input(deck_path("small"))
+    t = 0.29
+    o = seeded(O, d, t)
+    rng = np.random.default_rng(7)
+    for f in (4, 5):
+        o.field(f)[...] = rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape)
+    R = [o.field(4).copy(), o.field(5).copy()]
+    V0 = o.V.copy()
+    O.lib().hsto_linsolve(o.h, 1500.0, t)
+    with GPU(d) as g:
+        g.set_V(V0)
+        g.upload(hst.RHS1, R[0])
+        g.upload(hst.RHS2, R[1])
+        g.linsolve(1500.0, t)
+        Vg = g.V
+    assert rel(Vg, o.V) <= 1e-12
+
+
+@pytest.mark.parametrize("exact", [False, True])
+@pytest.mark.parametrize("precision,tol", [(hst.FP64, 1e-12), (hst.MIXED, 1e-5)])
+def test_one_step(O, exact, precision, tol):
+    d = dataclasses.replace(read_input(deck_path("small")), exact_shift=exact)
+    o = O.run_deck(d, nsteps=1)
+    g = hst.run_deck(d, precision, nsteps=1)
+    try:
+        assert g.time == o.time
+        assert rel(g.V, o.V) <= tol
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("name", ["small", "small_s2x"])
+def test_golden_fields(name):
+    """tests/regression.sh: 50 steps vs the reference's stored fields at 1e-10."""
+    d = read_input(deck_path(name))
+    g = hst.run_deck(d, hst.FP64)
+    try:
+        t, Vr, _ = cplio.read_field(f"{GOLDEN}/reference/{name}.fld")
+        assert g.nsteps == 50 and g.time == t
+        assert cplio.compare_fields(Vr, g.V) <= 1e-10
+    finally:
+        g.close()
+
+
+KELVIN = {"kelvin": 5e-4, "kelvin_exact": 1e-6, "kelvin_s2const": 1e-3, "kelvin_s2osc": 1e-3,
+          "kelvin_s2mean": 1e-3, "kelvin_stretched": 1e-3}
+
+
+def _solver_factory(precision=hst.FP64):
+    made = []
+
+    def make(deck):
+        for s in made:
+            s.close()
+        made.clear()
+        s = hst.Solver(deck, precision)
+        made.append(s)
+        return s
+
+    return make, made
+
+
+@pytest.mark.parametrize("name", list(KELVIN))
+def test_kelvin(name):
+    d = read_input(deck_path(name))
+    make, made = _solver_factory()
+    try:
+        err_v, err_eta, n = closed_forms.kelvin(make, d)
+    finally:
+        for s in made:
+            s.close()
+    assert err_v <= KELVIN[name], err_v
+    if err_eta is not None:
+        assert err_eta <= KELVIN[name], err_eta
+
+
+def test_taylor_green():
+    make, made = _solver_factory()
+    try:
+        worst = closed_forms.taylor_green(make, read_input(deck_path("pressure")))
+    finally:
+        for s in made:
+            s.close()
+    assert worst <= 1e-4, worst
+
+
+def test_forcing():
+    make, made = _solver_factory()
+    try:
+        r, errv = closed_forms.forcing(make, read_input(deck_path("pressure")))
+    finally:
+        for s in made:
+            s.close()
+    assert r <= 1e-2 and errv <= 1e-8
+
+
+def test_outstats(O):
+    d = read_input(deck_path("pressure"))
+    o = seeded(O, d, 0.0)
+    o.field(1)[0, d.nz, :] = 0.3 + 0.2 * np.sin(2 * np.pi * o.y / d.ly)
+    o.field(3)[0, d.nz, :] = -0.15 + 0.25 * np.cos(2 * np.pi * o.y / d.ly)
+    o.fill_all_ghosts(0.0)
+    ref = o.outstats()
+    with GPU(d) as g:
+        g.set_V(o.V)
+        g.time = 0.0
+        got = g.outstats()
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_chunked_transforms_identical(O):
+    """The transforms in y chunks (the memory-bounded path) give the same products."""
+    d = read_input(deck_path("small"))
+    V = hst.initial_field(d)
+    res = []
+    for chunk in (0, 7):
+        with GPU(d, hst.FP64, chunk_rows=chunk) as g:
+            g.set_V(V)
+            g.fill_all_ghosts(0.0)
+            g.nonlinear()
+            res.append(np.stack([g.product(p) for p in range(1, 7)]))
+    assert np.array_equal(res[0], res[1])
+
+
+def test_config1_fp64_matches_oracle(O):
+    """The correctness deck (nx = nz = 32, ny = 65, Re = 1000, 100 steps of 0.005):
+    per-field relative L2 and the energy and dissipation series vs the oracle at 1e-10."""
+    d = read_input(os.path.join(ROOT, "homogenenousshearturbulence_amd", "decks", "config1.in"))
+    o = O.Oracle(d)
+    o.initial_field()
+    o.fill_all_ghosts(0.0)
+    g = hst.Solver(d, hst.FP64)
+    try:
+        g.initial_field()
+        g.fill_all_ghosts(0.0)
+        worst_series = 0.0
+        for step in range(d.nstep):
+            o.timestep(d.deltat)
+            g.timestep(d.deltat)
+            if step % 10 == 9:
+                so, sg = o.outstats(), g.outstats()
+                worst_series = max(worst_series, abs(sg[0] - so[0]) / so[0], abs(sg[1] - so[1]) / so[1])
+        assert g.time == o.time
+        Vg, Vo = g.V, o.V
+        for c in range(3):
+            assert refops.rel_l2(Vg[c][..., 2:-2], Vo[c][..., 2:-2]) <= 1e-10, c
+        assert worst_series <= 1e-10
+    finally:
+        g.close()
