@@ -247,6 +247,8 @@ __global__ void __launch_bounds__((ZCfg<R, NZD>::NT))
            Cx<R> *__restrict__ Hv, Dims D, int row0, int nrows, const Cx<R> *__restrict__ tw) {
   constexpr int G = ZCfg<R, NZD>::G, T = ZCfg<R, NZD>::T, LS = NZD + 1;
   __shared__ Cx<R> lds[G * LS];
+  __shared__ Cx<R> twl[NZD];  // the twiddles: lanes of a stage read scattered entries
+  for (int k = threadIdx.x; k < NZD; k += blockDim.x) twl[k] = tw[k];
   const int c = blockIdx.z, rowc = blockIdx.y, row = row0 + rowc, ix0 = blockIdx.x * G;
   const double2 *src = (c == 0 ? V1 : (c == 1 ? V2 : V3)) + (size_t)row * D.ncol + (size_t)ix0 * D.NZm;
   const int nz = D.nz;
@@ -263,7 +265,7 @@ __global__ void __launch_bounds__((ZCfg<R, NZD>::NT))
   }
   __syncthreads();
   const int g = threadIdx.x / T, t = threadIdx.x % T;
-  fft_line<R, NZD, T, +1>(lds + g * LS, t, tw);
+  fft_line<R, NZD, T, +1>(lds + g * LS, t, twl);
   Cx<R> *dst = Hv + ((size_t)c * nrows + rowc) * NZD * D.NXp;
   for (int e = threadIdx.x; e < G * NZD; e += blockDim.x) {
     const int j = e % G, zp = e / G;
@@ -279,6 +281,8 @@ __global__ void __launch_bounds__((ZCfg<R, NZD>::NT))
            const Cx<R> *__restrict__ tw) {
   constexpr int G = ZCfg<R, NZD>::G, T = ZCfg<R, NZD>::T, LS = NZD + 1;
   __shared__ Cx<R> lds[G * LS];
+  __shared__ Cx<R> twl[NZD];
+  for (int k = threadIdx.x; k < NZD; k += blockDim.x) twl[k] = tw[k];
   const int p = blockIdx.z, rowc = blockIdx.y, row = row0 + rowc, ix0 = blockIdx.x * G;
   const Cx<R> *src = Hp + ((size_t)p * nrows + rowc) * NZD * D.NXp;
   for (int e = threadIdx.x; e < G * NZD; e += blockDim.x) {
@@ -287,7 +291,7 @@ __global__ void __launch_bounds__((ZCfg<R, NZD>::NT))
   }
   __syncthreads();
   const int g = threadIdx.x / T, t = threadIdx.x % T;
-  fft_line<R, NZD, T, -1>(lds + g * LS, t, tw);
+  fft_line<R, NZD, T, -1>(lds + g * LS, t, twl);
   Cx<R> *dst = P0 + (size_t)p * pstride + (size_t)row * D.ncol + (size_t)ix0 * D.NZm;
   const int nz = D.nz;
   for (int j = 0; j < G; j++) {
@@ -387,6 +391,174 @@ __global__ void __launch_bounds__((XCfg<R, M>::NT))
   }
 }
 
+// ------------------------------------------------------------------ k_xline
+
+// The x pass for power-of-two lengths M >= 64: T threads per line (T = M/8,
+// at most 64), E = M/T elements per thread, one wavefront per block.  A
+// thread holds the elements t + T e (e < E) of a line in registers; the
+// Stockham stages exchange through LDS only between stages, and the last
+// stage of a transform leaves the elements t + T e in the registers of
+// thread t again, which is where the first stage of the next transform
+// reads them: the products are formed in registers between the inverse and
+// the forward transforms.  LDS lines are padded by one element per eight.
+template <int M>
+struct XL {
+  static constexpr int T = (M / 8 < 64) ? M / 8 : 64;
+  static constexpr int E = M / T;
+  static constexpr int LPB = 64 / T;
+  static constexpr int PL = M + M / 8;
+};
+
+__device__ __forceinline__ int padx(int p) { return p + (p >> 3); }
+
+template <typename R, int M, int SIGN, int ST, int S>
+__device__ __forceinline__ void xl_fft(Cx<R> (&v)[XL<M>::E], Cx<R> *L, int t, const Cx<R> *__restrict__ tw) {
+  constexpr int T = XL<M>::T, E = XL<M>::E;
+  if constexpr (ST < n_stages(M)) {
+    constexpr int RAD = radix_at(M, ST);
+    constexpr int Ls = M / (S * RAD);
+    constexpr int NB = E / RAD;
+    constexpr bool FIRST = (S == 1), LAST = (S * RAD == M);
+    Cx<R> c[NB][RAD];
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      if constexpr (FIRST) {
+#pragma unroll
+        for (int n2 = 0; n2 < RAD; n2++) c[j][n2] = v[j + NB * n2];
+      } else {
+        const int b = t + T * j, q = b % S, n1 = b / S;
+#pragma unroll
+        for (int n2 = 0; n2 < RAD; n2++) c[j][n2] = L[padx(q + S * (n1 + Ls * n2))];
+      }
+    }
+    __syncthreads();  // every read of L done before it is written again
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      dft<R, SIGN, RAD>(c[j]);
+      if constexpr (!LAST) {
+        const int b = t + T * j, n1 = b / S;
+#pragma unroll
+        for (int k2 = 1; k2 < RAD; k2++) {
+          Cx<R> w = tw[n1 * k2 * S];
+          if (SIGN > 0) w.y = -w.y;
+          c[j][k2] = c[j][k2] * w;
+        }
+      }
+    }
+    if constexpr (LAST) {
+#pragma unroll
+      for (int j = 0; j < NB; j++)
+#pragma unroll
+        for (int k2 = 0; k2 < RAD; k2++) v[j + NB * k2] = c[j][k2];
+    } else {
+#pragma unroll
+      for (int j = 0; j < NB; j++) {
+        const int b = t + T * j, q = b % S, n1 = b / S;
+#pragma unroll
+        for (int k2 = 0; k2 < RAD; k2++) L[padx(q + S * k2 + S * RAD * n1)] = c[j][k2];
+      }
+      __syncthreads();
+      xl_fft<R, M, SIGN, ST + 1, S * RAD>(v, L, t, tw);
+    }
+  }
+}
+
+// Z(k) = (X(k) + conj X(M-k)) + i exp(i pi k/M) (X(k) - conj X(M-k)) at k = t + T e:
+// the half-length complex input of the length-2M inverse real transform.
+template <typename R, int M>
+__device__ __forceinline__ void xl_c2r_load(Cx<R> (&dst)[XL<M>::E], const Cx<R> *__restrict__ src, bool ok, int t,
+                                            int nx, const Cx<R> *__restrict__ twh) {
+  constexpr int T = XL<M>::T, E = XL<M>::E;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int k = t + T * e;
+    Cx<R> a{(R)0, (R)0}, b{(R)0, (R)0};
+    if (ok && k <= nx) a = src[k];
+    if (k == 0) a.y = (R)0;  // c2r: the imaginary part of the kx = 0 bin is ignored
+    if (ok && k > 0 && M - k <= nx) b = conj(src[M - k]);
+    const Cx<R> ep = conj(twh[k]);  // exp(+i pi k / M)
+    dst[e] = (a + b) + mul_i(ep * (a - b));
+  }
+}
+
+template <typename R, int M, int MODE>
+__global__ void __launch_bounds__(64)
+    k_xline(const Cx<R> *__restrict__ Hv, Cx<R> *__restrict__ Hp, Dims D, int nrows, const Cx<R> *__restrict__ twx,
+            const Cx<R> *__restrict__ twh, const double *__restrict__ dyl, double *__restrict__ cfl_part,
+            int want_cfl, int row0) {
+  constexpr int T = XL<M>::T, E = XL<M>::E, LPB = XL<M>::LPB, PL = XL<M>::PL;
+  __shared__ Cx<R> lds[LPB * PL];
+  __shared__ Cx<R> twl[M];  // twiddles: the lanes of a stage read scattered entries
+  for (int k = threadIdx.x; k < M; k += 64) twl[k] = twx[k];
+  __syncthreads();
+  const int g = threadIdx.x / T, t = threadIdx.x % T;
+  const int NZD = D.nzd, nx = D.nx;
+  const int line = blockIdx.x * LPB + g;
+  const bool ok = line < NZD * nrows;
+  const int rowc = ok ? line / NZD : 0, zp = ok ? line - rowc * NZD : 0;
+  Cx<R> *L = lds + g * PL;
+  Cx<R> u[E], v[E], w[E];
+  // C2R pre-processing of u, v, w straight from the x modes in HBM
+  xl_c2r_load<R, M>(u, Hv + (((size_t)0 * nrows + rowc) * NZD + zp) * D.NXp, ok, t, nx, twh);
+  xl_c2r_load<R, M>(v, Hv + (((size_t)1 * nrows + rowc) * NZD + zp) * D.NXp, ok, t, nx, twh);
+  xl_c2r_load<R, M>(w, Hv + (((size_t)2 * nrows + rowc) * NZD + zp) * D.NXp, ok, t, nx, twh);
+  xl_fft<R, M, +1, 0, 1>(u, L, t, twl);
+  xl_fft<R, M, +1, 0, 1>(v, L, t, twl);
+  xl_fft<R, M, +1, 0, 1>(w, L, t, twl);
+  // u[e] = x(2k) + i x(2k+1) at k = t + T e
+  if (want_cfl) {
+    double cm = 0.0;
+    const int iy = row0 + rowc - 2;
+    if (ok && iy >= 0 && iy < D.ny) {
+      const double idx = 1.0 / D.dx, idy = 1.0 / dyl[iy], idz = 1.0 / D.dz;
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        const double a = fabs((double)u[e].x) * idx + fabs((double)v[e].x) * idy + fabs((double)w[e].x) * idz;
+        const double b = fabs((double)u[e].y) * idx + fabs((double)v[e].y) * idy + fabs((double)w[e].y) * idz;
+        cm = fmax(cm, fmax(a, b));
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) cm = fmax(cm, __shfl_xor(cm, o));
+    if (threadIdx.x == 0) cfl_part[blockIdx.x] = cm;
+  }
+  if constexpr (MODE == 2) return;
+  const R factor = (R)D.factor;
+  constexpr int NP = (MODE == 0 ? 6 : 3);
+  // unrolled: with p known at compile time no register array is selected at
+  // run time (a run-time select of arrays would put them in scratch memory)
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    Cx<R> z[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      if constexpr (MODE == 1) {
+        const Cx<R> a = (p == 0 ? u[e] : (p == 1 ? v[e] : w[e]));
+        z[e] = Cx<R>{a.x * factor, a.y * factor};
+      } else {
+        const Cx<R> a = (p == 0 || p == 3 || p == 5) ? u[e] : (p == 1 || p == 4 ? v[e] : w[e]);
+        const Cx<R> b = (p == 0) ? u[e] : ((p == 1 || p == 3) ? v[e] : w[e]);
+        z[e] = Cx<R>{a.x * b.x * factor, a.y * b.y * factor};
+      }
+    }
+    xl_fft<R, M, -1, 0, 1>(z, L, t, twl);
+    // R2C post-processing needs Z(k) and Z(M - k)
+#pragma unroll
+    for (int e = 0; e < E; e++) L[padx(t + T * e)] = z[e];
+    __syncthreads();
+    Cx<R> *dst = Hp + (((size_t)p * nrows + rowc) * NZD + zp) * D.NXp;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int k = t + T * e;
+      if (ok && k <= nx) {
+        const Cx<R> Z = z[e], Zc = conj(L[padx((M - k) & (M - 1))]);
+        const Cx<R> ep = twh[k];  // exp(-i pi k / M)
+        const Cx<R> sm = (Z + Zc) * (R)0.5, df = (Z - Zc) * (R)0.5;
+        dst[k] = sm - mul_i(ep * df);
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 
 template <typename R, int NZD>
@@ -412,20 +584,35 @@ static int run_zfft(const Dims &D, int np, const void *Hp, void *P0, size_t pstr
 template <typename R, int M>
 static int run_xpass(const Dims &D, int mode, const void *Hv, void *Hp, int row0, int nrows, const Tables<R> &tb,
                      const double *dyl, double *cfl_part, int want_cfl, int *n_part, hipStream_t s) {
-  typedef XCfg<R, M> C;
-  if (D.nzd % C::G) return hipErrorInvalidValue;
-  dim3 grid(D.nzd / C::G, nrows);
-  if (n_part) *n_part = grid.x * grid.y;
-  if (mode == 0)
-    hipLaunchKernelGGL((k_xpass<R, M, 0>), grid, dim3(C::NT), 0, s, (const Cx<R> *)Hv, (Cx<R> *)Hp, D, row0, nrows,
-                       tb.twx, tb.twh, dyl, cfl_part, want_cfl);
-  else if (mode == 1)
-    hipLaunchKernelGGL((k_xpass<R, M, 1>), grid, dim3(C::NT), 0, s, (const Cx<R> *)Hv, (Cx<R> *)Hp, D, row0, nrows,
-                       tb.twx, tb.twh, dyl, cfl_part, want_cfl);
-  else
-    hipLaunchKernelGGL((k_xpass<R, M, 2>), grid, dim3(C::NT), 0, s, (const Cx<R> *)Hv, (Cx<R> *)Hp, D, row0, nrows,
-                       tb.twx, tb.twh, dyl, cfl_part, 1);
-  return hipGetLastError();
+  if constexpr ((M & (M - 1)) == 0 && M >= 64) {
+    typedef XL<M> C;
+    const int lines = D.nzd * nrows;
+    dim3 grid((lines + C::LPB - 1) / C::LPB);
+    if (n_part) *n_part = grid.x;
+#define LX(MD, WC)                                                                                                 \
+  hipLaunchKernelGGL((k_xline<R, M, MD>), grid, dim3(64), 0, s, (const Cx<R> *)Hv, (Cx<R> *)Hp, D, nrows, tb.twx, \
+                     tb.twh, dyl, cfl_part, WC, row0)
+    if (mode == 0) LX(0, want_cfl);
+    else if (mode == 1) LX(1, want_cfl);
+    else LX(2, 1);
+#undef LX
+    return hipGetLastError();
+  } else {
+    typedef XCfg<R, M> C;
+    if (D.nzd % C::G) return hipErrorInvalidValue;
+    dim3 grid(D.nzd / C::G, nrows);
+    if (n_part) *n_part = grid.x * grid.y;
+    if (mode == 0)
+      hipLaunchKernelGGL((k_xpass<R, M, 0>), grid, dim3(C::NT), 0, s, (const Cx<R> *)Hv, (Cx<R> *)Hp, D, row0, nrows,
+                         tb.twx, tb.twh, dyl, cfl_part, want_cfl);
+    else if (mode == 1)
+      hipLaunchKernelGGL((k_xpass<R, M, 1>), grid, dim3(C::NT), 0, s, (const Cx<R> *)Hv, (Cx<R> *)Hp, D, row0, nrows,
+                         tb.twx, tb.twh, dyl, cfl_part, want_cfl);
+    else
+      hipLaunchKernelGGL((k_xpass<R, M, 2>), grid, dim3(C::NT), 0, s, (const Cx<R> *)Hv, (Cx<R> *)Hp, D, row0, nrows,
+                         tb.twx, tb.twh, dyl, cfl_part, 1);
+    return hipGetLastError();
+  }
 }
 
 #define HST_FOR_SIZES(X) X(6) X(12) X(24) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536)

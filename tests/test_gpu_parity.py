@@ -116,6 +116,23 @@ def test_products(O, name, precision, tol):
             assert rel(Pg, P0[p]) <= tol, p
 
 
+@pytest.mark.parametrize("nx", [42, 84, 170, 340, 682])
+@pytest.mark.parametrize("precision,tol", [(hst.FP64, 1e-12), (hst.MIXED, 2e-5)])
+def test_products_power_of_two_lengths(O, nx, precision, tol):
+    """Products through the register-resident x pass (x lengths 2 nxd = 128 ..
+    2048, i.e. nxd = 64 .. 1024) vs the oracle on thin decks."""
+    d = dataclasses.replace(read_input(deck_path("small")), nx=nx, nz=4, ny=12, kpeak=3.0)
+    assert d.nxd in (64, 128, 256, 512, 1024)
+    o = seeded(O, d, 0.17)
+    o.nonlinear(mode=0)
+    P0 = o.products()
+    with GPU(d, precision) as g:
+        g.set_V(o.V)
+        g.nonlinear()
+        for p in range(6):
+            assert rel(g.product(p + 1), P0[p]) <= tol, p
+
+
 @pytest.mark.parametrize("exact", [False, True])
 @pytest.mark.parametrize("s2", [False, True])
 def test_rhs_and_shift(O, exact, s2):
