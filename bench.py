@@ -1,5 +1,5 @@
 """RK3 steps/s of the hst substep on one MI355X at nx = nz = 256, ny = 513
-(the metric deck, homogenenousshearturbulence_amd/decks/config3.in: Re 8000,
+(the metric deck, homogenenousshearturbulence_amd/decks/box256_y513_shift.in: Re 8000,
 exact_shift, dt = 0.001): the mixed-precision form (FP32 transforms, products
 and derivative right-hand sides; FP64 solves, Crank-Nicolson operators, mean
 mode and phase) against the FP64 form, the relative L2 error of the mixed
@@ -29,17 +29,17 @@ from homogenenousshearturbulence_amd.deck import read_input  # noqa: E402
 
 METRIC = "RK3 steps/s, nx=nz=256 ny=513: mixed vs FP64 at stated L2 error; % HBM peak"
 HBM_PEAK_GBS = 8000.0
-DECK = os.path.join(ROOT, "homogenenousshearturbulence_amd", "decks", "config3.in")
+DECK = os.path.join(ROOT, "homogenenousshearturbulence_amd", "decks", "box256_y513_shift.in")
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_form(deck, precision, V0, warmup, steps, acc_steps, prof_steps=2):
+def run_form(deck, precision, V0, warmup, steps, acc_steps, prof_steps=2, chunk=0):
     """Times `steps` RK3 steps after `warmup`, runs on to `acc_steps` steps in
     total for the accuracy statement, then times `prof_steps` more per kernel."""
-    s = hst.Solver(deck, precision)
+    s = hst.Solver(deck, precision, chunk)
     try:
         s.set_V(V0)
         s.zero(hst.OLD1)
@@ -100,7 +100,7 @@ def cpu_baseline(deck, threads):
     sub = time.perf_counter() - t0
     del o
     return {"value": 1.0 / (3.0 * sub), "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"one RK3 substep of config3 (nx=nz=256, ny=513) timed in {sub:.1f} s with {threads} threads, "
+            "sample": f"one RK3 substep of box256_y513_shift (nx=nz=256, ny=513) timed in {sub:.1f} s with {threads} threads, "
                       "steps/s = 1/(3 x substep time)"}
 
 
@@ -117,6 +117,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--deck", default=DECK)
+    ap.add_argument("--chunk", type=int, default=0, help="rows per transform chunk (0: all rows)")
+    ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -128,11 +130,20 @@ def main():
     V0 = hst.initial_field(deck)
     log(f"seeded field in {time.perf_counter() - t0:.1f} s")
     acc = max(args.accuracy_steps, args.warmup + args.steps)
+    if args.precision != "both":
+        # a single form, for sweeps: no accuracy statement
+        p = hst.MIXED if args.precision == "mixed" else hst.FP64
+        r = run_form(deck, p, V0, args.warmup, args.steps, 0, chunk=args.chunk)
+        tab = r["table"]
+        print(json.dumps({"precision": args.precision, "chunk": r["chunk"], "steps_per_s": args.steps / r["elapsed"],
+                          "kernels": {x["kernel"]: round(x["ms"] / x["launches"], 4) for x in tab},
+                          "kernel_ms_per_step": {x["kernel"]: round(x["ms"] / 2, 3) for x in tab}}), flush=True)
+        return
     log("FP64 form")
-    r64 = run_form(deck, hst.FP64, V0, args.warmup, args.steps, acc)
+    r64 = run_form(deck, hst.FP64, V0, args.warmup, args.steps, acc, chunk=args.chunk)
     log(f"FP64: {args.steps / r64['elapsed']:.3f} steps/s")
     log("mixed form")
-    rmx = run_form(deck, hst.MIXED, V0, args.warmup, args.steps, acc)
+    rmx = run_form(deck, hst.MIXED, V0, args.warmup, args.steps, acc, chunk=args.chunk)
     log(f"mixed: {args.steps / rmx['elapsed']:.3f} steps/s")
     del V0
     # accuracy of the mixed form against the FP64 form after `acc` steps
@@ -190,7 +201,7 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic: seeded divergence-free random field, E(k) ~ k^4 exp(-2 (k/12)^2), u_rms = 1",
         "config": {
-            "workload": "config3: nx=nz=256 (257 x 513 Fourier columns), ny=513, nxd=512 (1024 x points), nzd=768, "
+            "workload": "box256_y513_shift: nx=nz=256 (257 x 513 Fourier columns), ny=513, nxd=512 (1024 x points), nzd=768, "
                         "Re=8000, S=1, exact_shift, dt=0.001; mixed = FP32 transforms/products/derivative RHS, "
                         "FP64 solves/CN/mean mode/phase",
             "fp64_steps_per_s": round(sps_64, 4),

@@ -1,4 +1,4 @@
-"""Writes the decks of the five benchmark configurations as hst.in files.
+"""Writes the five decks as hst.in files.
 
 Every deck: alfa0 = beta0 = 1, ly = 2 pi, S = 1, uniform y, fixed step
 (cflmax = 0), seed 1, a fresh seeded field (restart_from_file = .false.),
@@ -22,11 +22,11 @@ TWO_PI = 2.0 * math.pi
 
 CONFIGS = {
     # name: (nx=nz, ny, re, kpeak, exact_shift, deltat, nstep, note)
-    "config1": (32, 65, 1000.0, 4.0, False, 0.005, 100, "correctness deck: FP64 vs the CPU oracle"),
-    "config2": (128, 257, 4000.0, 8.0, False, 0.002, 200, "FP64 on one GPU"),
-    "config3": (256, 513, 8000.0, 12.0, True, 0.001, 100, "the metric deck: mixed vs FP64"),
-    "config4": (512, 1025, 20000.0, 16.0, False, 5e-4, 20, "mixed precision at the largest one-GPU grid"),
-    "config5": (32, 65, 1000.0, 4.0, False, 0.005, 100, "coarse run of the learned closure"),
+    "box32_y65": (32, 65, 1000.0, 4.0, False, 0.005, 100, "correctness deck: FP64 vs the CPU oracle"),
+    "box128_y257": (128, 257, 4000.0, 8.0, False, 0.002, 200, "FP64 on one GPU"),
+    "box256_y513_shift": (256, 513, 8000.0, 12.0, True, 0.001, 100, "the metric deck: mixed vs FP64"),
+    "box512_y1025": (512, 1025, 20000.0, 16.0, False, 5e-4, 20, "mixed precision at the largest one-GPU grid"),
+    "les32_y65": (32, 65, 1000.0, 4.0, False, 0.005, 100, "coarse run of the learned closure"),
 }
 
 

@@ -304,10 +304,10 @@ def test_chunked_transforms_identical(O):
     assert np.array_equal(res[0], res[1])
 
 
-def test_config1_fp64_matches_oracle(O):
+def test_box32_fp64_100_steps_match_oracle(O):
     """The correctness deck (nx = nz = 32, ny = 65, Re = 1000, 100 steps of 0.005):
     per-field relative L2 and the energy and dissipation series vs the oracle at 1e-10."""
-    d = read_input(os.path.join(ROOT, "homogenenousshearturbulence_amd", "decks", "config1.in"))
+    d = read_input(os.path.join(ROOT, "homogenenousshearturbulence_amd", "decks", "box32_y65.in"))
     o = O.Oracle(d)
     o.initial_field()
     o.fill_all_ghosts(0.0)

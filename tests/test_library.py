@@ -64,7 +64,7 @@ def test_rejects_out_of_scope_decks():
 
 
 def test_benchmark_decks_have_unit_rms_velocity():
-    for k in (1, 2, 3, 4):
-        d = read_input(os.path.join(ROOT, "homogenenousshearturbulence_amd", "decks", f"config{k}.in"))
+    for name in ("box32_y65", "box128_y257", "box256_y513_shift", "box512_y1025", "les32_y65"):
+        d = read_input(os.path.join(ROOT, "homogenenousshearturbulence_amd", "decks", name + ".in"))
         assert abs(hst.initial_energy(d) - 3.0) < 1e-12
         assert d.alfa0 == 1.0 and d.beta0 == 1.0 and d.cflmax == 0.0
