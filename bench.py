@@ -104,6 +104,22 @@ def cpu_baseline(deck, threads):
                       "steps/s = 1/(3 x substep time)"}
 
 
+def copy_probe(mib):
+    """Five launches of a plain elementwise copy (read n bytes, write n bytes):
+    what FETCH_SIZE and WRITE_SIZE read for it scales the counters of the
+    solver's kernels to bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    x = torch.rand(n, device="cuda:0", dtype=torch.float32)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 5}), flush=True)
+
+
 def step_bytes(table, steps):
     return sum(r["bytes"] for r in table) / steps
 
@@ -119,7 +135,12 @@ def main():
     ap.add_argument("--deck", default=DECK)
     ap.add_argument("--chunk", type=int, default=0, help="rows per transform chunk (0: all rows)")
     ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run five plain device copies of MIB MiB (the HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -134,17 +135,19 @@ static int run(const char *name, double bytes, F launch) {
   return 0;
 }
 
+int profiled(const char *name, double bytes, const std::function<int()> &launch) {
+  return run(name, bytes, [&] { return launch(); });
+}
+
 // ------------------------------------------------------------------ clock
 
-static double s2_of(double t) {
-  const hst_deck &d = S.deck;
+static double s2_of_deck(const hst_deck &d, double t) {
   if ((d.s2_amplitude == 0.0 && d.s2_mean == 0.0) || t < d.s2_start) return 0.0;
   if (d.s2_period > 0.0) return d.s2_mean + d.s2_amplitude * std::sin(TWOPI / d.s2_period * (t - d.s2_start));
   return d.s2_mean + d.s2_amplitude;
 }
 
-static double s2_integral(double t1, double t2) {
-  const hst_deck &d = S.deck;
+static double s2_integral_deck(const hst_deck &d, double t1, double t2) {
   if ((d.s2_amplitude == 0.0 && d.s2_mean == 0.0) || t2 <= d.s2_start) return 0.0;
   const double a = std::max(t1, d.s2_start), b = t2;
   if (d.s2_period > 0.0) {
@@ -155,6 +158,9 @@ static double s2_integral(double t1, double t2) {
   return (d.s2_mean + d.s2_amplitude) * (b - a);
 }
 
+static double s2_of(double t) { return s2_of_deck(S.deck, t); }
+static double s2_integral(double t1, double t2) { return s2_integral_deck(S.deck, t1, t2); }
+
 // Fortran MODULO of reals (fmod, then the sign fix)
 static double modulo_r(double a, double p) {
   double r = std::fmod(a, p);
@@ -162,9 +168,32 @@ static double modulo_r(double a, double p) {
   return r;
 }
 
-static void shear_shifts(double t, double &sx, double &sz) {
-  sx = modulo_r(S.deck.S * t * S.deck.ly, S.g.lx);
-  sz = modulo_r(s2_integral(S.deck.s2_start, t) * S.deck.ly, S.g.lz);
+void deck_shifts(const hst_deck &d, double lx, double lz, double t, double &sx, double &sz) {
+  sx = modulo_r(d.S * t * d.ly, lx);
+  sz = modulo_r(s2_integral_deck(d, d.s2_start, t) * d.ly, lz);
+}
+
+static void shear_shifts(double t, double &sx, double &sz) { deck_shifts(S.deck, S.g.lx, S.g.lz, t, sx, sz); }
+
+// ------------------------------------------------------------------ the closure's view of the state
+
+static StressHook g_hook = nullptr;
+static void *g_hook_ctx = nullptr;
+
+void set_stress_hook(StressHook f, void *ctx) {
+  g_hook = f;
+  g_hook_ctx = ctx;
+}
+
+bool state_view(StateView &v) {
+  if (!S.live) return false;
+  v.mixed = S.mixed;
+  v.D = S.D;
+  v.deck = S.deck;
+  for (int i = 0; i < 9; i++) v.F[i] = S.F[i];
+  v.P = S.P;
+  v.stream = S.stream;
+  return true;
 }
 
 // ------------------------------------------------------------------ pieces
@@ -336,17 +365,10 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
   grid_setup(S.deck, S.g, S.y, S.dyl, S.der);
   if (!fft_size_supported(S.g.nzd)) return fail("unsupported z transform size nzd = " + std::to_string(S.g.nzd));
   if (!fft_size_supported(S.g.nxd)) return fail("unsupported x transform size nxd = " + std::to_string(S.g.nxd));
+  S.D = make_dims(*deck, S.g);
   Dims &D = S.D;
-  D.nx = deck->nx; D.ny = deck->ny; D.nz = deck->nz;
-  D.NXm = S.g.nxm; D.NZm = S.g.nzm; D.NYg = S.g.nyg; D.ncol = S.g.ncol;
-  D.nxd = S.g.nxd; D.nzd = S.g.nzd;
-  D.NXp = (D.NXm + 15) / 16 * 16;
-  D.alfa0 = deck->alfa0; D.beta0 = deck->beta0; D.ni = S.g.ni; D.S = deck->S;
-  D.factor = S.g.factor; D.dx = S.g.dx; D.dz = S.g.dz;
   d0_factor(S.der, D.ny, S.f0);
-  std::memcpy(S.fc.rec, S.f0.rec, sizeof(S.fc.rec));
-  S.fc.c1m4 = S.f0.c1m4; S.fc.c1m3 = S.f0.c1m3; S.fc.d11 = S.f0.d11; S.fc.d12 = S.f0.d12; S.fc.c10 = S.f0.c10;
-  S.fc.c2m3 = S.f0.c2m3; S.fc.d21 = S.f0.d21; S.fc.d22 = S.f0.d22; S.fc.c20 = S.f0.c20; S.fc.c21 = S.f0.c21;
+  S.fc = d0_const(S.f0);
 
   HIPCHK(hipStreamCreate(&S.stream));
   const size_t nf = (size_t)D.NYg * D.ncol;
@@ -547,6 +569,8 @@ int hst_gpu_timestep(double deltat, double *time, double *cfl) {
     if (!S.deck.linear) rc = nonlinear_pass(0, want, &c);
     else if (want) rc = nonlinear_pass(2, true, &c);  // linear: the transform only for the CFL number
     if (rc) return rc;
+    // an attached closure adds its modelled stresses to the six products
+    if (!S.deck.linear && g_hook && (rc = g_hook(g_hook_ctx, t))) return rc;
     const double dt_sub = 2.0 / ode[0] * deltat;
     const bool fuse_shift = !S.deck.exact_shift;
     const double dgx = S.deck.S * dt_sub, dgz = s2_integral(t, t + dt_sub);

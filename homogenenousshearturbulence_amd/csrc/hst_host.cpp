@@ -339,6 +339,25 @@ double initial_energy(const hst_deck &d) {
   return q2;
 }
 
+Dims make_dims(const hst_deck &d, const hst_grid_info &g) {
+  Dims D;
+  D.nx = d.nx; D.ny = d.ny; D.nz = d.nz;
+  D.NXm = g.nxm; D.NZm = g.nzm; D.NYg = g.nyg; D.ncol = g.ncol;
+  D.nxd = g.nxd; D.nzd = g.nzd;
+  D.NXp = (D.NXm + 15) / 16 * 16;
+  D.alfa0 = d.alfa0; D.beta0 = d.beta0; D.ni = g.ni; D.S = d.S;
+  D.factor = g.factor; D.dx = g.dx; D.dz = g.dz;
+  return D;
+}
+
+D0Const d0_const(const D0Factor &f) {
+  D0Const c;
+  std::memcpy(c.rec, f.rec, sizeof(c.rec));
+  c.c1m4 = f.c1m4; c.c1m3 = f.c1m3; c.d11 = f.d11; c.d12 = f.d12; c.c10 = f.c10;
+  c.c2m3 = f.c2m3; c.d21 = f.d21; c.d22 = f.d22; c.c20 = f.c20; c.c21 = f.c21;
+  return c;
+}
+
 }  // namespace hst
 
 extern "C" int hst_initial_energy(const hst_deck *deck, double *q2) {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <complex>
+#include <functional>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -116,5 +117,30 @@ int launch_to_device_layout(const Dims &D, const double2 *ref, double2 *dev, hip
 int launch_to_ref_layout(const Dims &D, const double2 *dev, double2 *ref, hipStream_t s);
 int launch_product_to_field(const Dims &D, int mixed, const void *P, double2 *dst, hipStream_t s);
 
+
+// ---------------------------------------------------------------- shared helpers
+
+Dims make_dims(const hst_deck &d, const hst_grid_info &g);
+D0Const d0_const(const D0Factor &f);
+// The shear shifts (sx, sz) of a deck at time t (src/hst_derivatives.f90:123-128).
+void deck_shifts(const hst_deck &d, double lx, double lz, double t, double &sx, double &sz);
+
+// ---------------------------------------------------------------- the live state, as the closure sees it
+
+struct StateView {
+  int mixed;
+  Dims D;
+  hst_deck deck;
+  double2 *F[9];  // fields by id (HST_V1 .. HST_SCRATCH)
+  void *P;        // the six product spectra [6][NYg][ncol], Cx<float> in the mixed form
+  hipStream_t stream;
+};
+bool state_view(StateView &v);
+// Called by hst_gpu_timestep after the product pass of every substep, with
+// the time of the substep; a nonzero return stops the step.
+typedef int (*StressHook)(void *ctx, double time);
+void set_stress_hook(StressHook f, void *ctx);
+// Runs a launch on the state's stream, timed when the state's profiling is on.
+int profiled(const char *name, double bytes, const std::function<int()> &launch);
 
 }  // namespace hst

@@ -1001,3 +1001,294 @@ int hsto_max_threads(void) {
   return 1;
 #endif
 }
+
+/* ================================================================== learned closure
+ *
+ * The learned subgrid closure has no counterpart in the reference: these
+ * functions restate this library's own definitions (the filter, the samples,
+ * the network and its gradient) so that the GPU kernels have a checker.
+ *
+ * Pinning:
+ *   filter weights     closed forms: unit sum, the 4:1 trapezoid [1/8 1/4 1/4 1/4 1/8],
+ *                      exactness on linear data, the top-hat transfer function
+ *   to/from physical   the round trip and single-mode closed forms (tests/)
+ *   network gradient   central finite differences of the FP64 loss
+ *   bf16 rounding      torch's CPU bfloat16 conversion (round to nearest even)
+ */
+
+/* Top-hat filter in y of width ly/nyc centred on coarse node k, applied to the
+ * piecewise-linear interpolant of the fine rows (uniform grids, same ly).
+ * Fine row j (may be -2, -1 or >= nyf: the shear-periodic images) gets
+ * weight (1/(b-a)) * integral over [a, b] of the hat function centred on j,
+ * [a, b] = k nyf/nyc -+ nyf/(2 nyc) in fine-row units.  Up to 8 taps from j0[k]. */
+static double hat_cdf(double x) {
+  if (x <= -1.0) return 0.0;
+  if (x <= 0.0) return 0.5 * (x + 1.0) * (x + 1.0);
+  if (x <= 1.0) return 1.0 - 0.5 * (1.0 - x) * (1.0 - x);
+  return 1.0;
+}
+
+int hsto_filter_weights(int nyf, int nyc, int *j0, double *w) {
+  if (nyc < 2 || nyf < nyc) return 1;
+  const double r = 0.5 * (double)nyf / (double)nyc;
+  for (int k = 0; k < nyc; k++) {
+    const double c = (double)k * (double)nyf / (double)nyc, a = c - r, b = c + r;
+    const int lo = (int)floor(a), hi = (int)ceil(b);
+    if (hi - lo + 1 > 8) return 2;
+    j0[k] = lo;
+    for (int t = 0; t < 8; t++) {
+      const int j = lo + t;
+      w[k * 8 + t] = (j <= hi) ? (hat_cdf(b - j) - hat_cdf(a - j)) / (b - a) : 0.0;
+    }
+  }
+  return 0;
+}
+
+/* Filter of a fine spectral field (reference layout, ghost rows filled) to the
+ * coarse grid: the Fourier columns |ix| <= nx_c, |iz| <= nz_c are kept (a
+ * sharp spectral box in x and z; alfa0, beta0 equal) and each coarse row is
+ * the weighted sum of fine rows above.  Coarse ghost rows are left to the
+ * caller. */
+void hsto_filter_field(const hsto *fo, const hsto *co, const double *fine, double *coarse) {
+  const cplx *F = (const cplx *)fine;
+  cplx *C = (cplx *)coarse;
+  int *j0 = (int *)malloc(sizeof(int) * co->ny);
+  double *w = (double *)malloc(sizeof(double) * 8 * co->ny);
+  hsto_filter_weights(fo->ny, co->ny, j0, w);
+  for (int ix = 0; ix <= co->nx; ix++)
+    for (int iz = -co->nz; iz <= co->nz; iz++)
+      for (int k = 0; k < co->ny; k++) {
+        cplx s = 0.0;
+        for (int t = 0; t < 8; t++) {
+          if (w[k * 8 + t] == 0.0) continue;
+          s += w[k * 8 + t] * F[FIDX(fo, ix, iz, j0[k] + t)];
+        }
+        C[FIDX(co, ix, iz, k)] = s;
+      }
+  free(j0);
+  free(w);
+}
+
+/* A spectral field to physical values at the interior rows:
+ * phys[iy][zp][x], x < 2 nxd, zp < nzd.  op 0: the field; 1: d/dx (times
+ * i alfa0 ix); 2: d/dz (times i beta0 iz).  The transforms of row_nonlinear. */
+void hsto_to_physical(const hsto *o, const double *field, int op, double *phys) {
+  const cplx *V = (const cplx *)field;
+  const int nzd = o->nzd, nx = o->nx, nz = o->nz, N = 2 * o->nxd;
+#pragma omp parallel
+  {
+    cplx *VVx = (cplx *)malloc(sizeof(cplx) * (size_t)nzd * (nx + 1));
+    const int L = nzd > N ? nzd : N;
+    cplx *line = (cplx *)malloc(sizeof(cplx) * 2 * L), *w2 = line + L;
+#pragma omp for
+    for (int iy = 0; iy < o->ny; iy++) {
+      for (int ix = 0; ix <= nx; ix++) {
+        for (int k = 0; k < nzd; k++) {
+          int iz = k <= nz ? k : (k >= nzd - nz ? k - nzd : 1 << 30);
+          cplx v = 0.0;
+          if (iz != 1 << 30) {
+            v = V[FIDX(o, ix, iz, iy)];
+            if (op == 1) v *= I * o->d.alfa0 * ix;
+            else if (op == 2) v *= I * o->d.beta0 * iz;
+          }
+          line[k] = v;
+        }
+        fft_run(&o->pz, line, w2, +1);
+        for (int k = 0; k < nzd; k++) VVx[(size_t)k * (nx + 1) + ix] = line[k];
+      }
+      for (int k = 0; k < nzd; k++) {
+        for (int j = 0; j < N; j++) line[j] = 0.0;
+        line[0] = creal(VVx[(size_t)k * (nx + 1)]);
+        for (int ix = 1; ix <= nx; ix++) {
+          line[ix] = VVx[(size_t)k * (nx + 1) + ix];
+          line[N - ix] = conj(line[ix]);
+        }
+        fft_run(&o->px, line, w2, +1);
+        double *out = phys + ((size_t)iy * nzd + k) * N;
+        for (int j = 0; j < N; j++) out[j] = creal(line[j]);
+      }
+    }
+    free(VVx);
+    free(line);
+  }
+}
+
+/* Physical values at the interior rows back to a spectral field (reference
+ * layout): times factor, real forward x transform, z transform, truncation
+ * to the kept modes.  Ghost rows are left to the caller. */
+void hsto_from_physical(const hsto *o, const double *phys, double *field) {
+  cplx *V = (cplx *)field;
+  const int nzd = o->nzd, nx = o->nx, nz = o->nz, N = 2 * o->nxd;
+#pragma omp parallel
+  {
+    cplx *HX = (cplx *)malloc(sizeof(cplx) * (size_t)nzd * (nx + 1));
+    const int L = nzd > N ? nzd : N;
+    cplx *line = (cplx *)malloc(sizeof(cplx) * 2 * L), *w2 = line + L;
+#pragma omp for
+    for (int iy = 0; iy < o->ny; iy++) {
+      for (int k = 0; k < nzd; k++) {
+        const double *in = phys + ((size_t)iy * nzd + k) * N;
+        for (int j = 0; j < N; j++) line[j] = in[j] * o->factor;
+        fft_run(&o->px, line, w2, -1);
+        for (int ix = 0; ix <= nx; ix++) HX[(size_t)k * (nx + 1) + ix] = line[ix];
+      }
+      for (int ix = 0; ix <= nx; ix++) {
+        for (int k = 0; k < nzd; k++) line[k] = HX[(size_t)k * (nx + 1) + ix];
+        fft_run(&o->pz, line, w2, -1);
+        for (int iz = -nz; iz <= nz; iz++) V[FIDX(o, ix, iz, iy)] = line[iz >= 0 ? iz : nzd + iz];
+      }
+    }
+    free(HX);
+    free(line);
+  }
+}
+
+/* ghost rows of an arbitrary reference-layout field (wrap_rows) */
+void hsto_fill_ghosts_of(const hsto *o, double *field, double sx, double sz) {
+  cplx *f = (cplx *)field;
+  const int ny = o->ny;
+  for (int ix = 0; ix <= o->nx; ix++)
+    for (int iz = -o->nz; iz <= o->nz; iz++) {
+      double a = -(o->d.alfa0 * ix * sx + o->d.beta0 * iz * sz);
+      cplx ph = cos(a) + I * sin(a);
+      cplx *c = f + FIDX(o, ix, iz, -2);
+      c[ny + 2] = c[0 + 2] * ph;
+      c[ny + 1 + 2] = c[1 + 2] * ph;
+      c[-1 + 2] = c[ny - 1 + 2] * conj(ph);
+      c[-2 + 2] = c[ny - 2 + 2] * conj(ph);
+    }
+}
+
+/* ------------------------------------------------------------------ the network
+ *
+ * 9 inputs (the filtered velocity gradients du_i/dx_j, i-major), two hidden
+ * layers of 64 tanh units, 6 outputs (tau_uu vv ww uv vw uw).  Parameters,
+ * one flat array: W1[64][9] b1[64] W2[64][64] b2[64] W3[6][64] b3[6].
+ * Inputs and targets are standardised: xh = (x - mx) * rx, yh = (y - my) * ry
+ * with norm = [mx(9) sx(9) my(6) sy(6)], rx = 1/sx, ry = 1/sy, in FP32.
+ * Loss: mean over samples and the 6 outputs of (out - yh)^2.
+ *
+ * bf = 1 rounds to bfloat16 (nearest even) where the GPU kernels feed the
+ * matrix cores: the standardised inputs, the weights, the hidden activations
+ * and the back-propagated errors; sums, biases and tanh stay in FP32/FP64. */
+#define MLP_NIN 9
+#define MLP_NH 64
+#define MLP_NOUT 6
+#define MLP_NP (64 * 9 + 64 + 64 * 64 + 64 + 6 * 64 + 6)
+#define OW1 0
+#define OB1 (64 * 9)
+#define OW2 (OB1 + 64)
+#define OB2 (OW2 + 64 * 64)
+#define OW3 (OB2 + 64)
+#define OB3 (OW3 + 6 * 64)
+
+int hsto_mlp_nparam(void) { return MLP_NP; }
+
+float hsto_bf16_round(float x) {
+  union { float f; uint32_t u; } v;
+  v.f = x;
+  if ((v.u & 0x7f800000u) == 0x7f800000u) return x; /* inf, nan: unchanged */
+  v.u += 0x7fffu + ((v.u >> 16) & 1u);
+  v.u &= 0xffff0000u;
+  return v.f;
+}
+
+static inline double bfr(double x, int bf) { return bf ? (double)hsto_bf16_round((float)x) : x; }
+static inline double f32r(double x, int bf) { return bf ? (double)(float)x : x; }
+
+/* Forward and (when grad) backward pass over n samples; returns the loss
+ * (sum over samples and outputs divided by n*6).  out: n*6 standardised
+ * outputs when not NULL. */
+static double mlp_pass(const double *par, const double *norm, int64_t n, const float *X, const float *Y, int bf,
+                       double *grad, double *out) {
+  double W1[64 * 9], W2[64 * 64], W3[6 * 64];
+  for (int i = 0; i < 64 * 9; i++) W1[i] = bfr(par[OW1 + i], bf);
+  for (int i = 0; i < 64 * 64; i++) W2[i] = bfr(par[OW2 + i], bf);
+  for (int i = 0; i < 6 * 64; i++) W3[i] = bfr(par[OW3 + i], bf);
+  const double *b1 = par + OB1, *b2 = par + OB2, *b3 = par + OB3;
+  float mx[9], rx[9], my[6], ry[6];
+  for (int k = 0; k < 9; k++) { mx[k] = (float)norm[k]; rx[k] = 1.0f / (float)norm[9 + k]; }
+  for (int k = 0; k < 6; k++) { my[k] = (float)norm[18 + k]; ry[k] = 1.0f / (float)norm[24 + k]; }
+  if (grad) memset(grad, 0, sizeof(double) * MLP_NP);
+  double loss = 0.0;
+  const double scale = 2.0 / ((double)n * 6.0);
+#pragma omp parallel reduction(+ : loss)
+  {
+    double *g = grad ? (double *)calloc(MLP_NP, sizeof(double)) : NULL;
+#pragma omp for schedule(static)
+    for (int64_t s = 0; s < n; s++) {
+      double x[9], h1[64], h2[64], h1b[64], h2b[64], o[6];
+      for (int k = 0; k < 9; k++) {
+        float xh = (X[s * 9 + k] - mx[k]) * rx[k];
+        x[k] = bf ? (double)hsto_bf16_round(xh) : (double)xh;
+      }
+      for (int j = 0; j < 64; j++) {
+        double z = 0.0;
+        for (int k = 0; k < 9; k++) z += W1[j * 9 + k] * x[k];
+        z = f32r(f32r(z, bf) + b1[j], bf);
+        h1[j] = tanh(z);
+        h1b[j] = bfr(h1[j], bf);
+      }
+      for (int j = 0; j < 64; j++) {
+        double z = 0.0;
+        for (int k = 0; k < 64; k++) z += W2[j * 64 + k] * h1b[k];
+        z = f32r(f32r(z, bf) + b2[j], bf);
+        h2[j] = tanh(z);
+        h2b[j] = bfr(h2[j], bf);
+      }
+      double dy[6];
+      for (int j = 0; j < 6; j++) {
+        double z = 0.0;
+        for (int k = 0; k < 64; k++) z += W3[j * 64 + k] * h2b[k];
+        o[j] = f32r(f32r(z, bf) + b3[j], bf);
+        if (out) out[s * 6 + j] = o[j];
+        if (Y) {
+          float yh = (Y[s * 6 + j] - my[j]) * ry[j];
+          double e = o[j] - (double)yh;
+          loss += e * e;
+          dy[j] = f32r(scale * e, bf);
+        }
+      }
+      if (!g || !Y) continue;
+      double dyb[6], d2[64], d2b[64], d1[64], d1b[64];
+      for (int j = 0; j < 6; j++) dyb[j] = bfr(dy[j], bf);
+      for (int k = 0; k < 64; k++) {
+        double a = 0.0;
+        for (int j = 0; j < 6; j++) a += W3[j * 64 + k] * dyb[j];
+        d2[k] = f32r(f32r(a, bf) * (1.0 - h2[k] * h2[k]), bf);
+        d2b[k] = bfr(d2[k], bf);
+      }
+      for (int k = 0; k < 64; k++) {
+        double a = 0.0;
+        for (int j = 0; j < 64; j++) a += W2[j * 64 + k] * d2b[j];
+        d1[k] = f32r(f32r(a, bf) * (1.0 - h1[k] * h1[k]), bf);
+        d1b[k] = bfr(d1[k], bf);
+      }
+      for (int j = 0; j < 6; j++) {
+        g[OB3 + j] += dy[j];
+        for (int k = 0; k < 64; k++) g[OW3 + j * 64 + k] += dyb[j] * h2b[k];
+      }
+      for (int j = 0; j < 64; j++) {
+        g[OB2 + j] += d2[j];
+        for (int k = 0; k < 64; k++) g[OW2 + j * 64 + k] += d2b[j] * h1b[k];
+        g[OB1 + j] += d1[j];
+        for (int k = 0; k < 9; k++) g[OW1 + j * 9 + k] += d1b[j] * x[k];
+      }
+    }
+    if (g) {
+#pragma omp critical
+      for (int i = 0; i < MLP_NP; i++) grad[i] += g[i];
+      free(g);
+    }
+  }
+  return loss / ((double)n * 6.0);
+}
+
+double hsto_mlp_grad(const double *par, const double *norm, int64_t n, const float *X, const float *Y, int bf,
+                     double *grad) {
+  return mlp_pass(par, norm, n, X, Y, bf, grad, NULL);
+}
+
+void hsto_mlp_forward(const double *par, const double *norm, int64_t n, const float *X, int bf, double *out) {
+  mlp_pass(par, norm, n, X, NULL, bf, NULL, out);
+}

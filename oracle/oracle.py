@@ -62,6 +62,19 @@ def lib():
         L.hsto_outstats_at.argtypes = [vp, d, dp]
         L.hsto_set_threads.argtypes = [i]
         L.hsto_max_threads.restype = i
+        # learned closure
+        L.hsto_filter_weights.restype = i
+        L.hsto_filter_weights.argtypes = [i, i, vp, vp]
+        L.hsto_filter_field.argtypes = [vp, vp, vp, vp]
+        L.hsto_to_physical.argtypes = [vp, vp, i, vp]
+        L.hsto_from_physical.argtypes = [vp, vp, vp]
+        L.hsto_fill_ghosts_of.argtypes = [vp, vp, d, d]
+        L.hsto_mlp_nparam.restype = i
+        L.hsto_bf16_round.restype = ctypes.c_float
+        L.hsto_bf16_round.argtypes = [ctypes.c_float]
+        L.hsto_mlp_grad.restype = d
+        L.hsto_mlp_grad.argtypes = [vp, vp, ctypes.c_int64, vp, vp, i, vp]
+        L.hsto_mlp_forward.argtypes = [vp, vp, ctypes.c_int64, vp, i, vp]
         _lib = L
     return _lib
 
@@ -187,3 +200,141 @@ def _reset_old(self) -> None:
 
 
 Oracle.reset_old = _reset_old
+
+
+# ---------------------------------------------------------------- learned closure
+
+
+def filter_weights(ny_fine: int, ny_coarse: int):
+    j0 = np.zeros(ny_coarse, dtype=np.int32)
+    w = np.zeros((ny_coarse, 8))
+    rc = lib().hsto_filter_weights(ny_fine, ny_coarse, j0.ctypes.data, w.ctypes.data)
+    if rc:
+        raise ValueError(f"filter weights: status {rc}")
+    return j0, w
+
+
+def filter_field(fine: "Oracle", coarse: "Oracle", f: np.ndarray) -> np.ndarray:
+    """The coarse filtered field (interior rows; ghosts zero) of a fine field."""
+    f = np.ascontiguousarray(f, dtype=np.complex128)
+    out = np.zeros(coarse.shape, dtype=np.complex128)
+    lib().hsto_filter_field(fine.h, coarse.h, f.ctypes.data, out.ctypes.data)
+    return out
+
+
+def to_physical(o: "Oracle", f: np.ndarray, op: int = 0) -> np.ndarray:
+    """Physical values [ny][nzd][2 nxd]; op 1: d/dx, 2: d/dz."""
+    f = np.ascontiguousarray(f, dtype=np.complex128)
+    out = np.zeros((o.deck.ny, o.info.nzd, 2 * o.info.nxd))
+    lib().hsto_to_physical(o.h, f.ctypes.data, op, out.ctypes.data)
+    return out
+
+
+def from_physical(o: "Oracle", phys: np.ndarray) -> np.ndarray:
+    phys = np.ascontiguousarray(phys, dtype=np.float64)
+    out = np.zeros(o.shape, dtype=np.complex128)
+    lib().hsto_from_physical(o.h, phys.ctypes.data, out.ctypes.data)
+    return out
+
+
+def fill_ghosts_of(o: "Oracle", f: np.ndarray, sx: float, sz: float) -> None:
+    assert f.flags.c_contiguous and f.dtype == np.complex128
+    lib().hsto_fill_ghosts_of(o.h, f.ctypes.data, sx, sz)
+
+
+def dy(o: "Oracle", f: np.ndarray, sx: float, sz: float) -> np.ndarray:
+    """d/dy of a field with ghost rows filled: the D0 solve of D1 f (KIND_DY)."""
+    o.field(8)[...] = f
+    o.line_solve(5, 0.0, 8, 7, sx, sz)
+    return o.field(7).copy()
+
+
+def gradients(o: "Oracle", U, sx: float, sz: float):
+    """The 9 physical gradient fields du_i/dx_j (i-major) of U (3 fields, ghosts valid)."""
+    g = []
+    for i in range(3):
+        g.append(to_physical(o, U[i], 1))
+        g.append(to_physical(o, dy(o, U[i], sx, sz), 0))
+        g.append(to_physical(o, U[i], 2))
+    return g
+
+
+def samples(o: "Oracle", U, Pbar, sx: float, sz: float):
+    """(x[n][9], y[n][6]) at every coarse grid point, index ((iy nzd + zp) 2 nxd + x)."""
+    g = gradients(o, U, sx, sz)
+    u = [to_physical(o, U[i]) for i in range(3)]
+    pairs = ((0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (0, 2))
+    tau = [to_physical(o, Pbar[p]) - u[a] * u[b] for p, (a, b) in enumerate(pairs)]
+    x = np.stack([a.ravel() for a in g], axis=1)
+    y = np.stack([a.ravel() for a in tau], axis=1)
+    return x, y
+
+
+def mlp_nparam() -> int:
+    return lib().hsto_mlp_nparam()
+
+
+def bf16_round(x: float) -> float:
+    return lib().hsto_bf16_round(x)
+
+
+def mlp_grad(params, norm, x, y, bf16: bool):
+    """(loss, gradient) of the mean squared standardised error; bf16 rounds
+    where the GPU kernels feed the matrix cores."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    nm = np.ascontiguousarray(norm, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    g = np.zeros(p.size)
+    loss = lib().hsto_mlp_grad(p.ctypes.data, nm.ctypes.data, x.shape[0], x.ctypes.data, y.ctypes.data, int(bf16),
+                               g.ctypes.data)
+    return loss, g
+
+
+def mlp_forward(params, norm, x, bf16: bool) -> np.ndarray:
+    """Standardised outputs [n][6]."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    nm = np.ascontiguousarray(norm, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros((x.shape[0], 6))
+    lib().hsto_mlp_forward(p.ctypes.data, nm.ctypes.data, x.shape[0], x.ctypes.data, int(bf16), out.ctypes.data)
+    return out
+
+
+def closure_stress(o: "Oracle", U, params, norm, sx: float, sz: float, bf16: bool = True):
+    """The modelled stress spectra (6 fields, ghosts at sx, sz) of U: gradients,
+    the network at every grid point, the forward transform.  `o` is a helper
+    state of the coarse deck (its fields 7 and 8 are used as work space)."""
+    g = gradients(o, U, sx, sz)
+    x = np.stack([a.ravel() for a in g], axis=1).astype(np.float32)
+    out = mlp_forward(params, norm, x, bf16)
+    sy = np.float32(norm[24:30]).astype(np.float32)
+    my = np.float32(norm[18:24]).astype(np.float32)
+    tau = (out.astype(np.float32) * sy + my).astype(np.float64)
+    shape = g[0].shape
+    T = []
+    for p in range(6):
+        f = from_physical(o, tau[:, p].reshape(shape))
+        fill_ghosts_of(o, f, sx, sz)
+        T.append(f)
+    return np.stack(T)
+
+
+def step_with_closure(o: "Oracle", helper: "Oracle", deltat: float, params, norm) -> None:
+    """hsto_timestep with the modelled stresses added to the products after
+    every product pass (the closure's hook in the substep)."""
+    L = lib()
+    t = o.time
+    for ode in RK_RAI:
+        o.nonlinear(0)
+        sx, sz = o.shear_shifts(t)
+        o.products()[...] += closure_stress(helper, o.V, params, norm, sx, sz)
+        c = (ctypes.c_double * 3)(*ode)
+        L.hsto_buildrhs_prepare(o.h, c, deltat, t)
+        L.hsto_buildrhs(o.h, c, 1)
+        L.hsto_buildrhs(o.h, c, 2)
+        dts = 2.0 / ode[0] * deltat
+        L.hsto_shear_shift(o.h, dts, t)
+        t = t + dts
+        L.hsto_linsolve(o.h, ode[0] / deltat, t)
+    o.time = t

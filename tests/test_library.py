@@ -8,12 +8,12 @@ import numpy as np
 import pytest
 
 from conftest import ROOT, deck_path
-from homogenenousshearturbulence_amd import hst
+from homogenenousshearturbulence_amd import closure, hst
 from homogenenousshearturbulence_amd.deck import read_input
 
 
-def header_functions():
-    text = open(os.path.join(ROOT, "include", "hst_gpu.h")).read()
+def header_functions(header="hst_gpu.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
     return sorted(set(re.findall(r"^int\s+(hst_\w+)\s*\(", text, re.M)))
 
 
@@ -24,6 +24,11 @@ def test_exports_every_declared_entry_point():
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(hst.EXPORTS)
+    cnames = header_functions("hst_closure.h")
+    assert len(cnames) >= 20
+    for n in cnames:
+        assert hasattr(closure.lib(), n), n
+    assert set(cnames) == set(closure.EXPORTS)
 
 
 @pytest.mark.parametrize("name", ["small", "kelvin_stretched", "pressure"])
