@@ -186,13 +186,15 @@ def main():
     avg_ms = dom["ms"] / dom["launches"]
     per_launch = dom["bytes"] / dom["launches"]
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    # HBM bytes per launch of that kernel from the FETCH_SIZE / WRITE_SIZE runs
+    # (profiles/traffic_box256_y513_shift_mixed.md), scaled by the copy calibration
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        try:
-            traffic = json.load(open(tpath)).get(dom["kernel"])
-        except Exception:
-            traffic = None
+        with open(tpath) as fh:
+            t = json.load(fh)
+        if t.get("workload") == os.path.basename(args.deck)[:-3] and t.get("form") == "mixed":
+            traffic = t.get("bytes_per_launch", {}).get(dom["kernel"])
     bytes_mx = step_bytes(rmx["table"], 2)
     bytes_64 = step_bytes(r64["table"], 2)
     kernels = {r["kernel"]: {"ms_per_launch": round(r["ms"] / r["launches"], 4),

@@ -37,7 +37,7 @@ static int fill_table(Cx<R> **dst, int n, double scale_den) {
 
 bool fft_size_supported(int n) {
   switch (n) {
-    case 6: case 12: case 24: case 48: case 64: case 96: case 128: case 192: case 256:
+    case 6: case 12: case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192: case 256:
     case 384: case 512: case 768: case 1024: case 1536:
       return true;
     default:
@@ -471,7 +471,7 @@ static int run_xpass(const Dims &D, int mode, const void *Hv, void *Hp, int row0
   }
 }
 
-#define HST_FOR_SIZES(X) X(6) X(12) X(24) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536)
+#define HST_FOR_SIZES(X) X(6) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536)
 
 template <typename R>
 static int zift_dispatch(const Dims &D, const double2 *const V[3], void *Hv, int row0, int nrows, const Tables<R> &tb,
