@@ -93,6 +93,24 @@ struct XCfg {
   static constexpr int NT = G * T;
 };
 
+// ------------------------------------------------------------------ XCD-aware block order
+
+// Workgroups are dealt to the 8 XCDs round-robin by their linear id, and each
+// XCD has its own L2.  The z kernels read (k_zfft) or write (k_zift) segments
+// of G x points that share 128-byte lines with the next x tile, so the linear
+// id is renumbered to give every XCD a contiguous range of tiles: neighbours
+// then meet in the same L2 instead of fetching the line twice.  Returns
+// (tile, row, component) for a grid (ntile, nrow, nc).
+__device__ __forceinline__ void xcd_tile(int &tile, int &row, int &comp) {
+  const int nt = gridDim.x, nr = gridDim.y, n = nt * nr * gridDim.z;
+  const int lid = blockIdx.x + nt * (blockIdx.y + nr * blockIdx.z);
+  const int per = n / 8, rem = n % 8, x = lid % 8, s = lid / 8;
+  const int l = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + s;
+  tile = l % nt;
+  row = (l / nt) % nr;
+  comp = l / (nt * nr);
+}
+
 // ------------------------------------------------------------------ k_zift
 
 // z inverse transform of the three velocity components for the rows of a
@@ -105,7 +123,9 @@ __global__ void __launch_bounds__((ZCfg<R, NZD>::NT))
   __shared__ Cx<R> lds[G * LS];
   __shared__ Cx<R> twl[NZD];  // the twiddles: lanes of a stage read scattered entries
   for (int k = threadIdx.x; k < NZD; k += blockDim.x) twl[k] = tw[k];
-  const int c = blockIdx.z, rowc = blockIdx.y, row = row0 + rowc, ix0 = blockIdx.x * G;
+  int tile, rowc, c;
+  xcd_tile(tile, rowc, c);
+  const int row = row0 + rowc, ix0 = tile * G;
   const double2 *src = (c == 0 ? V1 : (c == 1 ? V2 : V3)) + (size_t)row * D.ncol + (size_t)ix0 * D.NZm;
   const int nz = D.nz;
   for (int j = 0; j < G; j++) {
@@ -139,7 +159,9 @@ __global__ void __launch_bounds__((ZCfg<R, NZD>::NT))
   __shared__ Cx<R> lds[G * LS];
   __shared__ Cx<R> twl[NZD];
   for (int k = threadIdx.x; k < NZD; k += blockDim.x) twl[k] = tw[k];
-  const int p = blockIdx.z, rowc = blockIdx.y, row = row0 + rowc, ix0 = blockIdx.x * G;
+  int tile, rowc, p;
+  xcd_tile(tile, rowc, p);
+  const int row = row0 + rowc, ix0 = tile * G;
   const Cx<R> *src = Hp + ((size_t)p * nrows + rowc) * NZD * D.NXp;
   for (int e = threadIdx.x; e < G * NZD; e += blockDim.x) {
     const int j = e % G, zp = e / G;

@@ -281,85 +281,123 @@ __device__ __forceinline__ void border_solve(const double *rr, double c1m4, doub
 
 // ------------------------------------------------------------------ per-column solves
 
-__device__ __forceinline__ double coef(int kind, double lam, double ni, double kk, const double *der, int iy, int j) {
+template <int K>
+__device__ __forceinline__ double coefk(double lam, double ni, double kk, const double *der, int iy, int j) {
   const double d0 = DER(der, iy, 0, j);
-  if (kind == KD2V) {
+  if constexpr (K == KD2V) {
     const double d2 = DER(der, iy, 2, j), d4 = DER(der, iy, 3, j);
     return lam * (d2 - kk * d0) - ni * (d4 - 2.0 * kk * d2 + kk * kk * d0);
+  } else if constexpr (K == KETA) {
+    return lam * d0 - ni * (DER(der, iy, 2, j) - kk * d0);
+  } else if constexpr (K == KPOISSON) {
+    return DER(der, iy, 2, j) - kk * d0;
+  } else {
+    return d0;
   }
-  if (kind == KETA) return lam * d0 - ni * (DER(der, iy, 2, j) - kk * d0);
-  if (kind == KPOISSON) return DER(der, iy, 2, j) - kk * d0;
-  return d0;
+}
+
+// Row pipelines of the per-column recurrences: body(i, v) for i = 0..n-1
+// (ascending) or n-1..0 (descending) with v = load(i) issued PF rows ahead,
+// so that a wavefront keeps PF rows of loads in flight while its recurrence
+// runs.  n, i are uniform over the wavefront.
+template <int PF, class T, class LOAD, class BODY>
+__device__ __forceinline__ void rows_ascending(int n, LOAD load, BODY body) {
+  T buf[PF];
+#pragma unroll
+  for (int k = 0; k < PF; k++)
+    if (k < n) buf[k] = load(k);
+  int i = 0;
+  for (; i + PF <= n; i += PF) {
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+      const T v = buf[k];
+      if (i + k + PF < n) buf[k] = load(i + k + PF);
+      body(i + k, v);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PF; k++)
+    if (i + k < n) body(i + k, buf[k]);
+}
+
+template <int PF, class T, class LOAD, class BODY>
+__device__ __forceinline__ void rows_descending(int n, LOAD load, BODY body) {
+  rows_ascending<PF, T>(
+      n, [&](int k) { return load(n - 1 - k); }, [&](int k, const T &v) { body(n - 1 - k, v); });
 }
 
 struct PentaArgs {
-  int kind[2];
   const double2 *src[2];
   double2 *dst[2];
   PentaWS ws[2];
 };
 
+// a row of the elimination workspace, as the back substitution reads it
+struct WSRow {
+  C xr;
+  double b1, b2, u1, u2;
+};
+
+constexpr int PF_FWD = 8, PF_BWD = 4, PF_PENTA = 4;
+
 // One cyclic pentadiagonal line per lane (line_solve, src/hst_linsolve.f90:147-483,
 // one slab): forward elimination of rows 0..ny-3 with the rows generated from
 // der, record in registers, 2x2 border system, back substitution.
-__global__ void __launch_bounds__(256)
-    k_penta(Dims D, PentaArgs pa, double lam, const double *__restrict__ der, double sx, double sz) {
-  const int sys = blockIdx.y;
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= D.ncol) return;
-  const int kind = pa.kind[sys];
-  const double2 *__restrict__ src = pa.src[sys];
-  double2 *__restrict__ dst = pa.dst[sys];
-  const PentaWS ws = pa.ws[sys];
+template <int K>
+__device__ __forceinline__ void penta_line(const Dims &D, int col, double lam, const double *__restrict__ der,
+                                           const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                           const PentaWS &ws, double sx, double sz) {
   int ix, iz;
   col_ix_iz(D, col, ix, iz);
   const size_t nc = D.ncol;
   const int m = D.ny, mi = m - 2;
-  if (ix == 0 && iz == 0 && kind == KD2V) {
+  if (ix == 0 && iz == 0 && K == KD2V) {
     for (int i = 0; i < m; i++) st(dst + (size_t)(i + 2) * nc + col, C{0.0, 0.0});
     return;
   }
-  if (ix == 0 && iz == 0 && kind == KPOISSON) return;
+  if (ix == 0 && iz == 0 && K == KPOISSON) return;
   const double al = D.alfa0 * ix, be = D.beta0 * iz;
   const double kk = al * al + be * be, ni = D.ni;
   double u11 = 0, u12 = 0, p1 = 0, q1 = 0, u21 = 0, u22 = 0, p2 = 0, q2 = 0;
   double w1 = 0, w2 = 0, v1 = 0, v2 = 0, sp0 = 0, sq0 = 0, sp1 = 0, sq1 = 0;
   C x1{0, 0}, x2{0, 0}, sx0{0, 0}, sx1{0, 0};
-  for (int i = 0; i < mi; i++) {
-    double a[5];
+  rows_ascending<PF_PENTA, C>(
+      mi, [&](int i) { return ld(src + (size_t)(i + 2) * nc + col); },
+      [&](int i, const C &bin) {
+        double a[5];
 #pragma unroll
-    for (int j = 0; j < 5; j++) a[j] = coef(kind, lam, ni, kk, der, i, j - 2);
-    double p = 0.0, q = 0.0;
-    if (i == 0) { p = a[0]; q = a[1]; }
-    if (i == 1) q = a[0];
-    C bx = ld(src + (size_t)(i + 2) * nc + col);
-    if (i >= 2) {
-      const double l = a[0];
-      a[1] = a[1] - l * u21;
-      a[2] = a[2] - l * u22;
-      bx = bx - l * x2; p = p - l * p2; q = q - l * q2;
-    }
-    if (i >= 1) {
-      const double l = a[1];
-      a[2] = a[2] - l * u11;
-      a[3] = a[3] - l * u12;
-      bx = bx - l * x1; p = p - l * p1; q = q - l * q1;
-    }
-    const double rp = 1.0 / a[2];
-    const double t1 = a[3] * rp, t2 = a[4] * rp;
-    bx = bx * rp; p = p * rp; q = q * rp;
-    const size_t o = (size_t)i * nc + col;
-    ws.U1[o] = t1; ws.U2[o] = t2; ws.B1[o] = p; ws.B2[o] = q; st(ws.XR + o, bx);
-    double ww = -(u11 * w1 + u22 * w2);
-    if (i == 0) ww = 1.0;
-    double vv = -(u11 * v1 + u22 * v2);
-    if (i == 1) vv = 1.0;
-    sx0 = sx0 + ww * bx; sp0 = sp0 + ww * p; sq0 = sq0 + ww * q;
-    sx1 = sx1 + vv * bx; sp1 = sp1 + vv * p; sq1 = sq1 + vv * q;
-    w2 = w1; w1 = ww; v2 = v1; v1 = vv;
-    u21 = u11; u22 = u12; x2 = x1; p2 = p1; q2 = q1;
-    u11 = t1; u12 = t2; x1 = bx; p1 = p; q1 = q;
-  }
+        for (int j = 0; j < 5; j++) a[j] = coefk<K>(lam, ni, kk, der, i, j - 2);
+        double p = 0.0, q = 0.0;
+        if (i == 0) { p = a[0]; q = a[1]; }
+        if (i == 1) q = a[0];
+        C bx = bin;
+        if (i >= 2) {
+          const double l = a[0];
+          a[1] = a[1] - l * u21;
+          a[2] = a[2] - l * u22;
+          bx = bx - l * x2; p = p - l * p2; q = q - l * q2;
+        }
+        if (i >= 1) {
+          const double l = a[1];
+          a[2] = a[2] - l * u11;
+          a[3] = a[3] - l * u12;
+          bx = bx - l * x1; p = p - l * p1; q = q - l * q1;
+        }
+        const double rp = 1.0 / a[2];
+        const double t1 = a[3] * rp, t2 = a[4] * rp;
+        bx = bx * rp; p = p * rp; q = q * rp;
+        const size_t o = (size_t)i * nc + col;
+        ws.U1[o] = t1; ws.U2[o] = t2; ws.B1[o] = p; ws.B2[o] = q; st(ws.XR + o, bx);
+        double ww = -(u11 * w1 + u22 * w2);
+        if (i == 0) ww = 1.0;
+        double vv = -(u11 * v1 + u22 * v2);
+        if (i == 1) vv = 1.0;
+        sx0 = sx0 + ww * bx; sp0 = sp0 + ww * p; sq0 = sq0 + ww * q;
+        sx1 = sx1 + vv * bx; sp1 = sp1 + vv * p; sq1 = sq1 + vv * q;
+        w2 = w1; w1 = ww; v2 = v1; v1 = vv;
+        u21 = u11; u22 = u12; x2 = x1; p2 = p1; q2 = q1;
+        u11 = t1; u12 = t2; x1 = bx; p1 = p; q1 = q;
+      });
   double rr[25];
   rr[3] = sp0; rr[4] = sq0; rr[5] = w2 * u22 + w1 * u11; rr[6] = w1 * u12;
   rr[9] = sp1; rr[10] = sq1; rr[11] = v2 * u22 + v1 * u11; rr[12] = v1 * u12;
@@ -370,38 +408,58 @@ __global__ void __launch_bounds__(256)
   const int ga = m - 2;
   const C ph = cexpi(-(D.alfa0 * ix * sx + D.beta0 * iz * sz));
   C b1, b2;
-  border_solve(rr, coef(kind, lam, ni, kk, der, ga, -2), coef(kind, lam, ni, kk, der, ga, -1),
-               coef(kind, lam, ni, kk, der, ga, 0), coef(kind, lam, ni, kk, der, ga, 1),
-               coef(kind, lam, ni, kk, der, ga, 2), coef(kind, lam, ni, kk, der, ga + 1, -2),
-               coef(kind, lam, ni, kk, der, ga + 1, -1), coef(kind, lam, ni, kk, der, ga + 1, 0),
-               coef(kind, lam, ni, kk, der, ga + 1, 1), coef(kind, lam, ni, kk, der, ga + 1, 2), ph, sx0, sx1, xm2,
+  border_solve(rr, coefk<K>(lam, ni, kk, der, ga, -2), coefk<K>(lam, ni, kk, der, ga, -1),
+               coefk<K>(lam, ni, kk, der, ga, 0), coefk<K>(lam, ni, kk, der, ga, 1),
+               coefk<K>(lam, ni, kk, der, ga, 2), coefk<K>(lam, ni, kk, der, ga + 1, -2),
+               coefk<K>(lam, ni, kk, der, ga + 1, -1), coefk<K>(lam, ni, kk, der, ga + 1, 0),
+               coefk<K>(lam, ni, kk, der, ga + 1, 1), coefk<K>(lam, ni, kk, der, ga + 1, 2), ph, sx0, sx1, xm2,
                xm1, r25, r27, b1, b2);
   st(dst + (size_t)(m - 2 + 2) * nc + col, b1);
   st(dst + (size_t)(m - 1 + 2) * nc + col, b2);
   const C cph = conj(ph);
   const C yb1 = cph * b1, yb2 = cph * b2;
   C xk1 = b1, xk2 = b2;
-#pragma unroll 4
-  for (int i = mi - 1; i >= 0; i--) {
-    const size_t o = (size_t)i * nc + col;
-    const C xk = ld(ws.XR + o) - ws.B1[o] * yb1 - ws.B2[o] * yb2 - ws.U1[o] * xk1 - ws.U2[o] * xk2;
-    st(dst + (size_t)(i + 2) * nc + col, xk);
-    xk2 = xk1;
-    xk1 = xk;
-  }
+  rows_descending<PF_BWD, WSRow>(
+      mi,
+      [&](int i) {
+        const size_t o = (size_t)i * nc + col;
+        return WSRow{ld(ws.XR + o), ws.B1[o], ws.B2[o], ws.U1[o], ws.U2[o]};
+      },
+      [&](int i, const WSRow &w) {
+        const C xk = w.xr - w.b1 * yb1 - w.b2 * yb2 - w.u1 * xk1 - w.u2 * xk2;
+        st(dst + (size_t)(i + 2) * nc + col, xk);
+        xk2 = xk1;
+        xk1 = xk;
+      });
+}
+
+// blockIdx.y selects the system: K0 for the first, K1 for the second
+template <int K0, int K1>
+__global__ void __launch_bounds__(256)
+    k_penta(Dims D, PentaArgs pa, double lam, const double *__restrict__ der, double sx, double sz) {
+  const int sys = blockIdx.y;
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= D.ncol) return;
+  if (sys == 0) penta_line<K0>(D, col, lam, der, pa.src[0], pa.dst[0], pa.ws[0], sx, sz);
+  else penta_line<K1>(D, col, lam, der, pa.src[1], pa.dst[1], pa.ws[1], sx, sz);
 }
 
 int launch_penta(const Dims &D, int nsys, const int *kinds, double lambda, const double2 *const *src,
                  double2 *const *dst, const PentaWS *ws, const double *der, double sx, double sz, hipStream_t s) {
   PentaArgs pa;
   for (int k = 0; k < nsys; k++) {
-    pa.kind[k] = kinds[k];
     pa.src[k] = src[k];
     pa.dst[k] = dst[k];
     pa.ws[k] = ws[k];
   }
   dim3 grid((D.ncol + 255) / 256, nsys);
-  hipLaunchKernelGGL(k_penta, grid, dim3(256), 0, s, D, pa, lambda, der, sx, sz);
+#define L(A, B) hipLaunchKernelGGL((k_penta<A, B>), grid, dim3(256), 0, s, D, pa, lambda, der, sx, sz)
+  if (nsys == 2 && kinds[0] == KD2V && kinds[1] == KETA) L(KD2V, KETA);
+  else if (nsys == 1 && kinds[0] == KD2V) L(KD2V, KD2V);
+  else if (nsys == 1 && kinds[0] == KETA) L(KETA, KETA);
+  else if (nsys == 1 && kinds[0] == KPOISSON) L(KPOISSON, KPOISSON);
+  else return hipErrorInvalidValue;
+#undef L
   return hipGetLastError();
 }
 
@@ -410,23 +468,58 @@ int launch_penta(const Dims &D, int nsys, const int *kinds, double lambda, const
 // rows table: [i][L2 L1 RP U1 U2 B1 B2 W V]
 #define F0(i, k) f0[(i) * 9 + (k)]
 
-// Forward sweep of D0 x = r for one lane: r(i) from the functor, XR stored.
-template <class RHS>
-__device__ __forceinline__ void d0_forward(const Dims &D, const double *__restrict__ f0, RHS rhs, double2 *XR, int col,
-                                           C &x1, C &x2, C &sx0, C &sx1) {
+// Forward sweep of D0 x = r for one lane, XR stored; r = src (RHS 0) or the
+// D1 stencil of src (RHS 1, ghost rows filled).  Also returns the border
+// rows' right-hand sides r(ny-2), r(ny-1).
+template <int RHS>
+__device__ __forceinline__ void d0_forward(const Dims &D, const double *__restrict__ f0,
+                                           const double *__restrict__ der, const double2 *__restrict__ src,
+                                           double2 *__restrict__ XR, int col, C &x1, C &x2, C &sx0, C &sx1, C &r25,
+                                           C &r27) {
   const size_t nc = D.ncol;
-  const int mi = D.ny - 2;
+  const int m = D.ny, mi = m - 2;
   x1 = C{0, 0}; x2 = C{0, 0}; sx0 = C{0, 0}; sx1 = C{0, 0};
-  for (int i = 0; i < mi; i++) {
-    C bx = rhs(i);
-    if (i >= 2) bx = bx - F0(i, 0) * x2;
-    if (i >= 1) bx = bx - F0(i, 1) * x1;
-    bx = bx * F0(i, 2);
-    st(XR + (size_t)i * nc + col, bx);
-    sx0 = sx0 + F0(i, 7) * bx;
-    sx1 = sx1 + F0(i, 8) * bx;
-    x2 = x1;
-    x1 = bx;
+  C w[5];  // RHS 1: the window src(i-2 .. i+2)
+  if constexpr (RHS == 1) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) w[k + 1] = ld(src + (size_t)k * nc + col);
+  }
+  auto stencil = [&](int i) {
+    C r{0, 0};
+#pragma unroll
+    for (int j = -2; j <= 2; j++) r = r + DER(der, i, 1, j) * w[j + 2];
+    return r;
+  };
+  auto push = [&](const C &v) {
+    w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4]; w[4] = v;
+  };
+  rows_ascending<PF_FWD, C>(
+      mi, [&](int i) { return ld(src + (size_t)(i + (RHS == 0 ? 2 : 4)) * nc + col); },
+      [&](int i, const C &v) {
+        C bx;
+        if constexpr (RHS == 0) {
+          bx = v;
+        } else {
+          push(v);
+          bx = stencil(i);
+        }
+        if (i >= 2) bx = bx - F0(i, 0) * x2;
+        if (i >= 1) bx = bx - F0(i, 1) * x1;
+        bx = bx * F0(i, 2);
+        st(XR + (size_t)i * nc + col, bx);
+        sx0 = sx0 + F0(i, 7) * bx;
+        sx1 = sx1 + F0(i, 8) * bx;
+        x2 = x1;
+        x1 = bx;
+      });
+  if constexpr (RHS == 0) {
+    r25 = ld(src + (size_t)(m - 2 + 2) * nc + col);
+    r27 = ld(src + (size_t)(m - 1 + 2) * nc + col);
+  } else {
+    push(ld(src + (size_t)(m + 2) * nc + col));
+    r25 = stencil(m - 2);
+    push(ld(src + (size_t)(m + 3) * nc + col));
+    r27 = stencil(m - 1);
   }
 }
 
@@ -454,19 +547,8 @@ __global__ void __launch_bounds__(256)
   const int m = D.ny;
   double acc = 0.0;
   if (act) {
-    auto rhs = [&](int i) -> C {
-      if constexpr (RHS == 0) {
-        return ld(src + (size_t)(i + 2) * nc + col);
-      } else {
-        C r{0, 0};
-#pragma unroll
-        for (int j = -2; j <= 2; j++) r = r + DER(der, i, 1, j) * ld(src + (size_t)(i + j + 2) * nc + col);
-        return r;
-      }
-    };
-    C x1, x2, sx0, sx1;
-    d0_forward(D, f0, rhs, XR, col, x1, x2, sx0, sx1);
-    const C r25 = rhs(m - 2), r27 = rhs(m - 1);
+    C x1, x2, sx0, sx1, r25, r27;
+    d0_forward<RHS>(D, f0, der, src, XR, col, x1, x2, sx0, sx1, r25, r27);
     const C ph = cexpi(-(D.alfa0 * ix * sx + D.beta0 * iz * sz));
     C b1, b2;
     d0_border(fc, ph, x1, x2, sx0, sx1, r25, r27, b1, b2);
@@ -474,12 +556,11 @@ __global__ void __launch_bounds__(256)
     const C yb1 = cph * b1, yb2 = cph * b2;
     const double al = D.alfa0 * ix, be = D.beta0 * iz, kk = al * al + be * be;
     const bool m00 = (ix == 0 && iz == 0);
-    auto emit = [&](int i, C x) {
+    auto emit = [&](int i, C x, C eta) {
       const size_t o = (size_t)(i + 2) * nc + col;
       if constexpr (OUT == 0) {
         st(dst + o, x);
       } else if constexpr (OUT == 1) {
-        const C eta = ld(dst + o);
         if (m00) {
           st(dst + o, C{eta.x, 0.0});
           st(dst3 + o, C{eta.y, 0.0});
@@ -494,16 +575,24 @@ __global__ void __launch_bounds__(256)
         acc += wgt * (x.x * x.x + x.y * x.y);
       }
     };
-    emit(m - 2, b1);
-    emit(m - 1, b2);
+    auto eta_at = [&](int i) -> C {
+      if constexpr (OUT == 1) return ld(dst + (size_t)(i + 2) * nc + col);
+      else return C{0, 0};
+    };
+    emit(m - 2, b1, eta_at(m - 2));
+    emit(m - 1, b2, eta_at(m - 1));
     C xk1 = b1, xk2 = b2;
-#pragma unroll 4
-    for (int i = m - 3; i >= 0; i--) {
-      const C xk = ld(XR + (size_t)i * nc + col) - F0(i, 5) * yb1 - F0(i, 6) * yb2 - F0(i, 3) * xk1 - F0(i, 4) * xk2;
-      emit(i, xk);
-      xk2 = xk1;
-      xk1 = xk;
-    }
+    struct Row {
+      C xr, eta;
+    };
+    rows_descending<PF_BWD, Row>(
+        m - 2, [&](int i) { return Row{ld(XR + (size_t)i * nc + col), eta_at(i)}; },
+        [&](int i, const Row &r) {
+          const C xk = r.xr - F0(i, 5) * yb1 - F0(i, 6) * yb2 - F0(i, 3) * xk1 - F0(i, 4) * xk2;
+          emit(i, xk, r.eta);
+          xk2 = xk1;
+          xk1 = xk;
+        });
   }
   if constexpr (OUT == 2) {
     __shared__ double red[4];
@@ -554,10 +643,8 @@ __global__ void __launch_bounds__(256)
   col_ix_iz(D, col, ix, iz);
   const size_t nc = D.ncol;
   const int m = D.ny;
-  auto rhs = [&](int i) -> C { return ld(q + (size_t)(i + 2) * nc + col); };
-  C x1, x2, sx0, sx1;
-  d0_forward(D, f0, rhs, XR, col, x1, x2, sx0, sx1);
-  const C r25 = rhs(m - 2), r27 = rhs(m - 1);
+  C x1, x2, sx0, sx1, r25, r27;
+  d0_forward<0>(D, f0, der, q, XR, col, x1, x2, sx0, sx1, r25, r27);
   const C ph0 = cexpi(-(D.alfa0 * ix * sx0s + D.beta0 * iz * sz0s));
   C b1, b2;
   d0_border(fc, ph0, x1, x2, sx0, sx1, r25, r27, b1, b2);
@@ -574,21 +661,23 @@ __global__ void __launch_bounds__(256)
   // window W[k] = x'(i + 1 + k)
   C W0 = xpm2, W1 = xpm1, W2 = xp0 * ph1, W3 = xp1 * ph1;
   C xk1 = b1, xk2 = b2;
-  for (int i = m - 3; i >= 0; i--) {
-    const C xk = ld(XR + (size_t)i * nc + col) - F0(i, 5) * yb1 - F0(i, 6) * yb2 - F0(i, 3) * xk1 - F0(i, 4) * xk2;
-    xk2 = xk1;
-    xk1 = xk;
-    const C xp = xk * cexpi(th * y[i + 2]);
-    const int r = i + 2;
-    C acc{0, 0};
-    acc = acc + DER(der, r, 0, -2) * xp;
-    acc = acc + DER(der, r, 0, -1) * W0;
-    acc = acc + DER(der, r, 0, 0) * W1;
-    acc = acc + DER(der, r, 0, 1) * W2;
-    acc = acc + DER(der, r, 0, 2) * W3;
-    st(q + (size_t)(r + 2) * nc + col, acc);
-    W3 = W2; W2 = W1; W1 = W0; W0 = xp;
-  }
+  rows_descending<PF_BWD, C>(
+      m - 2, [&](int i) { return ld(XR + (size_t)i * nc + col); },
+      [&](int i, const C &xr) {
+        const C xk = xr - F0(i, 5) * yb1 - F0(i, 6) * yb2 - F0(i, 3) * xk1 - F0(i, 4) * xk2;
+        xk2 = xk1;
+        xk1 = xk;
+        const C xp = xk * cexpi(th * y[i + 2]);
+        const int r = i + 2;
+        C acc{0, 0};
+        acc = acc + DER(der, r, 0, -2) * xp;
+        acc = acc + DER(der, r, 0, -1) * W0;
+        acc = acc + DER(der, r, 0, 0) * W1;
+        acc = acc + DER(der, r, 0, 1) * W2;
+        acc = acc + DER(der, r, 0, 2) * W3;
+        st(q + (size_t)(r + 2) * nc + col, acc);
+        W3 = W2; W2 = W1; W1 = W0; W0 = xp;
+      });
   // W = x'(0..3); ghosts below: x'(-1) = x'(ny-1) conj(ph1), x'(-2) = x'(ny-2) conj(ph1)
   const C g1 = xpm1 * conj(ph1), g2 = xpm2 * conj(ph1);
   C acc{0, 0};

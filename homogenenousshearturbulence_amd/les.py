@@ -58,7 +58,9 @@ class Plan:
 
 
 def default_plan(quick: bool = False) -> Plan:
-    dns = read_input(os.path.join(DECKS, "box128_y257.in"))
+    # box128_y257 at half its step: at dt = 0.002 the grown turbulence passes
+    # the RK3 stability limit (CFL ~ 1.8 for a 5 u_rms peak) within t < 1
+    dns = dataclasses.replace(read_input(os.path.join(DECKS, "box128_y257.in")), deltat=0.001)
     les = read_input(os.path.join(DECKS, "les32_y65.in"))
     p = Plan(dns=dns, les=les)
     if quick:
@@ -86,11 +88,16 @@ def run_dns(plan: Plan, log=print):
         t_start = _time.perf_counter()
         for t in events:
             _steps_to(s, t, plan.dns.deltat)
-            s.nonlinear()
+            cfl = s.nonlinear(want_cfl=True) * plan.dns.deltat
+            st = s.outstats()
+            log(f"[les] DNS t = {s.time:.3f}: energy {st[0]:.5f}, dissipation {st[1]:.5f}, uv {st[2]:.5f}, "
+                f"CFL {cfl:.3f}")
+            if not np.all(np.isfinite(st)):
+                raise hst.HstError(f"the DNS diverged by t = {s.time}")
             c.filter(s.time)
             if t in plan.t_train:
                 n = c.sample(s.time)
-                log(f"[les] DNS t = {s.time:.3f}: {n} samples")
+                log(f"[les]   {n} samples")
             if t in plan.record_times:
                 records[t] = c.velocity()
         log(f"[les] DNS to t = {s.time:.3f} in {_time.perf_counter() - t_start:.1f} s")
