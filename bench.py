@@ -114,10 +114,15 @@ def copy_probe(mib):
     x = torch.rand(n, device="cuda:0", dtype=torch.float32)
     y = torch.empty_like(x)
     torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
     for _ in range(5):
         torch.mul(x, 1.0, out=y)
+    b.record()
     torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 5}), flush=True)
+    ms = a.elapsed_time(b) / 5
+    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 5,
+                      "ms_per_launch": ms, "GBps": 8 * n / (ms * 1e-3) / 1e9}), flush=True)
 
 
 def step_bytes(table, steps):

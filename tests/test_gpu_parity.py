@@ -133,6 +133,37 @@ def test_products_power_of_two_lengths(O, nx, precision, tol):
             assert rel(g.product(p + 1), P0[p]) <= tol, p
 
 
+@pytest.mark.parametrize("nz", [16, 32, 64, 128, 256])
+@pytest.mark.parametrize("precision,tol", [(hst.FP64, 1e-12), (hst.MIXED, 2e-5)])
+def test_products_z_lengths(O, nz, precision, tol):
+    """Products vs the oracle for z transform lengths nzd = 48 .. 768 (thin x, y)."""
+    d = dataclasses.replace(read_input(deck_path("small")), nx=6, nz=nz, ny=12, kpeak=3.0)
+    assert d.nzd == 3 * nz
+    o = seeded(O, d, 0.11)
+    o.nonlinear(mode=0)
+    P0 = o.products()
+    with GPU(d, precision) as g:
+        g.set_V(o.V)
+        g.nonlinear()
+        for p in range(6):
+            assert rel(g.product(p + 1), P0[p]) <= tol, p
+
+
+@pytest.mark.parametrize("n,ny", [(64, 33), (128, 17)])
+def test_one_step_wide_grids(O, n, ny):
+    """One FP64 step at x and z lengths of the larger decks (nxd 128, 256;
+    nzd 192, 384) against the oracle."""
+    d = dataclasses.replace(read_input(deck_path("small")), nx=n, nz=n, ny=ny, alfa0=1.0, beta0=1.0,
+                            ly=2 * np.pi, re=1000.0, kpeak=6.0, amplitude=0.002, deltat=0.001)
+    o = O.run_deck(d, nsteps=1)
+    g = hst.run_deck(d, hst.FP64, nsteps=1)
+    try:
+        assert g.time == o.time
+        assert rel(g.V, o.V) <= 1e-11
+    finally:
+        g.close()
+
+
 @pytest.mark.parametrize("exact", [False, True])
 @pytest.mark.parametrize("s2", [False, True])
 def test_rhs_and_shift(O, exact, s2):
