@@ -60,6 +60,12 @@ struct State {
   int chunk = 0;
   PentaWS pws[2];
   double2 *XR[4];
+  // factor tables of the D2V and ETA lines for up to three lambdas (the RK3
+  // substeps at a fixed step), built on first use when memory allows
+  void *ptab[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+  double ptab_lam[3] = {0.0, 0.0, 0.0};
+  int ptab_next = 0;
+  bool ptab_off = false;
   void *tables = nullptr;
   hipStream_t stream = nullptr;
   int64_t bytes_held = 0;
@@ -72,6 +78,8 @@ struct State {
 
 static State S;
 static int g_chunk_request = 0;
+
+static bool field_f32(int f) { return S.mixed && (f == HST_OLD1 || f == HST_OLD2); }
 
 static int fail(const std::string &what, int rc = 1) {
   set_error(what);
@@ -249,12 +257,13 @@ static int buildrhs(const double ode[3], double deltat, double time, int mode, b
   a.shift = shift ? 1 : 0;
   const void *P[6];
   for (int p = 0; p < 6; p++) P[p] = P_of(p);
-  // prepare: u, v, w and oldrhs in, rhs and oldrhs out (9 fields); a group adds
-  // its three product spectra and reads and writes rhs, oldrhs (8 fields)
-  const double ps = S.mixed ? 8.0 : 16.0, pf = (double)S.D.ny * S.D.ncol * ps;
-  double bytes = bytes_field() * 9.0;
-  if (mode == 3) bytes += 6.0 * pf;
-  else if (mode != 0) bytes = bytes_field() * 8.0 + 3.0 * pf;
+  // prepare: u, v, w and oldrhs in, rhs and oldrhs out; a group adds its three
+  // product spectra and reads and writes rhs, oldrhs.  Products and oldrhs are
+  // FP32 in the mixed form.
+  const double ps = S.mixed ? 8.0 : 16.0, pts = (double)S.D.ny * S.D.ncol;
+  double bytes = pts * (5.0 * 16.0 + 4.0 * ps);
+  if (mode == 3) bytes += 6.0 * pts * ps;
+  else if (mode != 0) bytes = pts * (4.0 * 16.0 + 4.0 * ps + 3.0 * ps);
   return run(mode == 3 ? "buildrhs" : (mode == 0 ? "buildrhs_prepare" : "buildrhs_group"), bytes, [&] {
     return launch_buildrhs(S.D, S.mixed, a, S.F[1], S.F[2], S.F[3], P, S.F[HST_RHS1], S.F[HST_RHS2], S.F[HST_OLD1],
                            S.F[HST_OLD2], S.d_der, S.d_y, S.stream);
@@ -266,22 +275,62 @@ static int exact_shift(double dt_sub, double time) {
   double sx0, sz0, sx1, sz1;
   shear_shifts(time, sx0, sz0);
   shear_shifts(time + dt_sub, sx1, sz1);
-  double2 *q[4] = {S.F[HST_RHS1], S.F[HST_RHS2], S.F[HST_OLD1], S.F[HST_OLD2]};
-  return run("exact_shift", 4.0 * 4.0 * bytes_field(), [&] {
-    return launch_exact_shift(S.D, 4, q, S.XR, S.d_f0, S.fc, S.d_der, S.d_y, dgx, dgz, sx0, sz0, sx1, sz1, S.stream);
+  void *q[4] = {S.F[HST_RHS1], S.F[HST_RHS2], S.F[HST_OLD1], S.F[HST_OLD2]};
+  const int f32[4] = {0, 0, S.mixed, S.mixed};
+  // per array: q in, the solution x out and back in (FP64), q out
+  const double qb = 2.0 * 2.0 * 16.0 + 2.0 * 2.0 * (S.mixed ? 8.0 : 16.0) + 4.0 * 2.0 * 16.0;
+  return run("exact_shift", qb / 16.0 * bytes_field(), [&] {
+    return launch_exact_shift(S.D, 4, q, f32, S.XR, S.d_f0, S.fc, S.d_der, S.d_y, dgx, dgz, sx0, sz0, sx1, sz1,
+                              S.stream);
   });
 }
 
 static int shear_shift(double dt_sub, double time) {
   if (S.deck.exact_shift) return exact_shift(dt_sub, time);
   const double dgx = S.deck.S * dt_sub, dgz = s2_integral(time, time + dt_sub);
-  return run("phase_shift", 8.0 * bytes_field(), [&] {
-    return launch_phase4(S.D, S.F[HST_RHS1], S.F[HST_RHS2], S.F[HST_OLD1], S.F[HST_OLD2], S.d_y, dgx, dgz, S.stream);
+  return run("phase_shift", (4.0 * 16.0 + 4.0 * (S.mixed ? 8.0 : 16.0)) / 16.0 * bytes_field(), [&] {
+    return launch_phase4(S.D, S.mixed, S.F[HST_RHS1], S.F[HST_RHS2], S.F[HST_OLD1], S.F[HST_OLD2], S.d_y, dgx, dgz,
+                         S.stream);
   });
 }
 
 static int ghosts(int nf, double2 *const *f, double sx, double sz) {
   return run("ghosts", nf * 8.0 * S.D.ncol * 16.0, [&] { return launch_ghosts(S.D, nf, f, sx, sz, S.stream); });
+}
+
+// The factor tables for lambda (D2V, ETA), or null when they do not fit.
+static const void *const *penta_tables(double lambda) {
+  if (S.ptab_off) return nullptr;
+  for (int k = 0; k < 3; k++)
+    if (S.ptab[k][0] && S.ptab_lam[k] == lambda) return S.ptab[k];
+  const int k = S.ptab_next;
+  const size_t tb = penta_table_bytes(S.D);
+  if (!S.ptab[k][0]) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b < 2 * tb + ((size_t)4 << 30)) {
+      S.ptab_off = true;
+      return nullptr;
+    }
+    if (hipMalloc(&S.ptab[k][0], tb) != hipSuccess || hipMalloc(&S.ptab[k][1], tb) != hipSuccess) {
+      (void)hipGetLastError();
+      S.ptab_off = true;
+      return nullptr;
+    }
+    S.bytes_held += 2 * (int64_t)tb;
+  }
+  const int kinds[2] = {HST_KIND_D2V, HST_KIND_ETA};
+  for (int j = 0; j < 2; j++) {
+    int rc = run("penta_factor", (double)tb, [&] {
+      return launch_penta_factor(S.D, kinds[j], lambda, S.d_der, S.ptab[k][j], S.stream);
+    });
+    if (rc) {
+      S.ptab_off = true;
+      return nullptr;
+    }
+  }
+  S.ptab_lam[k] = lambda;
+  S.ptab_next = (k + 1) % 3;
+  return S.ptab[k];
 }
 
 static int linsolve(double lambda, double time) {
@@ -290,8 +339,10 @@ static int linsolve(double lambda, double time) {
   const int kinds[2] = {HST_KIND_D2V, HST_KIND_ETA};
   const double2 *src[2] = {S.F[HST_RHS2], S.F[HST_RHS1]};
   double2 *dst[2] = {S.F[HST_V2], S.F[HST_V1]};
-  int rc = run("penta_d2v_eta", 2.0 * 8.0 * bytes_field(), [&] {
-    return launch_penta(S.D, 2, kinds, lambda, src, dst, S.pws, S.d_der, sx, sz, S.stream);
+  const void *const *tabs = penta_tables(lambda);
+  // algorithmic bytes of a line solve: rhs in, the eliminated rhs out and back in, solution out
+  int rc = run("penta_d2v_eta", 2.0 * 4.0 * bytes_field(), [&] {
+    return launch_penta(S.D, 2, kinds, lambda, src, dst, S.pws, tabs, S.d_der, sx, sz, S.stream);
   });
   if (rc) return rc;
   double2 *g2[1] = {S.F[HST_V2]};
@@ -315,11 +366,20 @@ static int check_field(int f) {
   return 0;
 }
 
+// a field the kernels take as FP64 complex (every one but oldrhs in the mixed form)
+static int check_field64(int f) {
+  if (int rc = check_field(f)) return rc;
+  if (field_f32(f)) return fail("oldrhs (fields 6, 7) is held in FP32 in the mixed form; use it through the step");
+  return 0;
+}
+
 static void release() {
   if (S.stream) (void)hipStreamSynchronize(S.stream);
   for (int i = 1; i <= 8; i++) if (S.F[i]) (void)hipFree(S.F[i]);
   void *ptrs[] = {S.P, S.Hv, S.Hp, S.ws, S.d_der, S.d_y, S.d_dyl, S.d_f0, S.d_part};
   for (void *p : ptrs) if (p) (void)hipFree(p);
+  for (auto &t : S.ptab)
+    for (void *p : t) if (p) (void)hipFree(p);
   if (S.tables) free_fft_tables(S.tables);
   for (auto &p : S.pending) { S.pool.push_back(p.a); S.pool.push_back(p.b); }
   for (auto e : S.pool) (void)hipEventDestroy(e);
@@ -375,9 +435,11 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
   const size_t cs = S.mixed ? sizeof(float2) : sizeof(double2);
   int64_t held = 0;
   for (int i = 1; i <= 8; i++) {
-    HIPCHK(hipMalloc(&S.F[i], nf * sizeof(double2)));
-    HIPCHK(hipMemset(S.F[i], 0, nf * sizeof(double2)));
-    held += nf * sizeof(double2);
+    // oldrhs holds the explicit nonlinear terms: FP32 in the mixed form, like the products
+    const size_t es = field_f32(i) ? sizeof(float2) : sizeof(double2);
+    HIPCHK(hipMalloc(&S.F[i], nf * es));
+    HIPCHK(hipMemset(S.F[i], 0, nf * es));
+    held += nf * es;
   }
   HIPCHK(hipMalloc(&S.P, 6 * nf * cs));
   HIPCHK(hipMemset(S.P, 0, 6 * nf * cs));
@@ -456,9 +518,10 @@ int hst_gpu_upload_field(int field, const double *host) {
   if (int rc = check_live()) return rc;
   if (int rc = check_field(field)) return rc;
   const size_t nf = (size_t)S.D.NYg * S.D.ncol;
-  double2 *stage = (double2 *)S.ws;
+  double2 *stage = (double2 *)S.ws, *wide = stage + nf;
   HIPCHK(hipMemcpyAsync(stage, host, nf * sizeof(double2), hipMemcpyHostToDevice, S.stream));
-  int rc = launch_to_device_layout(S.D, stage, S.F[field], S.stream);
+  int rc = launch_to_device_layout(S.D, stage, field_f32(field) ? wide : S.F[field], S.stream);
+  if (!rc && field_f32(field)) rc = launch_narrow(wide, (float2 *)S.F[field], nf, S.stream);
   if (rc) return hip_fail("layout", (hipError_t)rc);
   HIPCHK(hipStreamSynchronize(S.stream));
   return 0;
@@ -468,8 +531,10 @@ int hst_gpu_download_field(int field, double *host) {
   if (int rc = check_live()) return rc;
   if (int rc = check_field(field)) return rc;
   const size_t nf = (size_t)S.D.NYg * S.D.ncol;
-  double2 *stage = (double2 *)S.ws;
-  int rc = launch_to_ref_layout(S.D, S.F[field], stage, S.stream);
+  double2 *stage = (double2 *)S.ws, *wide = stage + nf;
+  int rc = 0;
+  if (field_f32(field)) rc = launch_widen((const float2 *)S.F[field], wide, nf, S.stream);
+  if (!rc) rc = launch_to_ref_layout(S.D, field_f32(field) ? wide : S.F[field], stage, S.stream);
   if (rc) return hip_fail("layout", (hipError_t)rc);
   HIPCHK(hipMemcpyAsync(host, stage, nf * sizeof(double2), hipMemcpyDeviceToHost, S.stream));
   HIPCHK(hipStreamSynchronize(S.stream));
@@ -479,13 +544,14 @@ int hst_gpu_download_field(int field, double *host) {
 int hst_gpu_zero_field(int field) {
   if (int rc = check_live()) return rc;
   if (int rc = check_field(field)) return rc;
-  HIPCHK(hipMemsetAsync(S.F[field], 0, (size_t)S.D.NYg * S.D.ncol * sizeof(double2), S.stream));
+  const size_t es = field_f32(field) ? sizeof(float2) : sizeof(double2);
+  HIPCHK(hipMemsetAsync(S.F[field], 0, (size_t)S.D.NYg * S.D.ncol * es, S.stream));
   return 0;
 }
 
 int hst_gpu_fill_ghosts(int field, double shift_x, double shift_z) {
   if (int rc = check_live()) return rc;
-  if (int rc = check_field(field)) return rc;
+  if (int rc = check_field64(field)) return rc;
   double2 *f[1] = {S.F[field]};
   return ghosts(1, f, shift_x, shift_z);
 }
@@ -510,7 +576,7 @@ int hst_gpu_roundtrip(void) {
 
 int hst_gpu_product_to_field(int p, int field) {
   if (int rc = check_live()) return rc;
-  if (int rc = check_field(field)) return rc;
+  if (int rc = check_field64(field)) return rc;
   if (p < 1 || p > 6) return fail("product index out of range (1..6)");
   int rc = launch_product_to_field(S.D, S.mixed, P_of(p - 1), S.F[field], S.stream);
   if (rc) return hip_fail("product_to_field", (hipError_t)rc);
@@ -535,8 +601,8 @@ int hst_gpu_shear_shift(double dt_sub, double time) {
 
 int hst_gpu_line_solve(int kind, double lambda, int src, int dst, double shift_x, double shift_z) {
   if (int rc = check_live()) return rc;
-  if (int rc = check_field(src)) return rc;
-  if (int rc = check_field(dst)) return rc;
+  if (int rc = check_field64(src)) return rc;
+  if (int rc = check_field64(dst)) return rc;
   if (src == dst) return fail("line_solve: src and dst must be different fields");
   if (kind == HST_KIND_D0 || kind == HST_KIND_DY) {
     return run(kind == HST_KIND_D0 ? "d0_solve" : "dy_solve", 4.0 * bytes_field(), [&] {
@@ -548,8 +614,8 @@ int hst_gpu_line_solve(int kind, double lambda, int src, int dst, double shift_x
   const int kinds[1] = {kind};
   const double2 *s[1] = {S.F[src]};
   double2 *d[1] = {S.F[dst]};
-  return run("penta", 8.0 * bytes_field(), [&] {
-    return launch_penta(S.D, 1, kinds, lambda, s, d, S.pws, S.d_der, shift_x, shift_z, S.stream);
+  return run("penta", 4.0 * bytes_field(), [&] {
+    return launch_penta(S.D, 1, kinds, lambda, s, d, S.pws, nullptr, S.d_der, shift_x, shift_z, S.stream);
   });
 }
 

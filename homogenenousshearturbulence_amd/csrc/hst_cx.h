@@ -40,6 +40,7 @@ template <typename R> HSTD Cx<R> cx_from(double2 v) { return Cx<R>{(R)v.x, (R)v.
 HSTD Cx<double> ld(const double2 *p) { double2 v = *p; return Cx<double>{v.x, v.y}; }
 HSTD void st(double2 *p, Cx<double> v) { *p = make_double2(v.x, v.y); }
 HSTD Cx<double> ld(const float2 *p) { float2 v = *p; return Cx<double>{(double)v.x, (double)v.y}; }
+HSTD void st(float2 *p, Cx<double> v) { *p = make_float2((float)v.x, (float)v.y); }
 HSTD Cx<float> ldf(const float2 *p) { float2 v = *p; return Cx<float>{v.x, v.y}; }
 
 #undef HSTD

@@ -80,16 +80,24 @@ struct RhsArgs {
   double ode0, ode1, ode2, deltat, s2now, dgx, dgz;
   int nonlinear, shift;
 };
-int launch_phase4(const Dims &D, double2 *R1, double2 *R2, double2 *O1, double2 *O2, const double *y, double dgx,
-                  double dgz, hipStream_t s);
+// oldrhs (O1, O2) is FP32 (float2) in the mixed form, FP64 otherwise.
+int launch_phase4(const Dims &D, int mixed, double2 *R1, double2 *R2, void *O1, void *O2, const double *y,
+                  double dgx, double dgz, hipStream_t s);
 int launch_buildrhs(const Dims &D, int mixed, const RhsArgs &a, const double2 *V1, const double2 *V2,
-                    const double2 *V3, const void *const P[6], double2 *R1, double2 *R2, double2 *O1, double2 *O2,
+                    const double2 *V3, const void *const P[6], double2 *R1, double2 *R2, void *O1, void *O2,
                     const double *der, const double *y, hipStream_t s);
 
 // Cyclic pentadiagonal solves with per-column rows (D2V, ETA, POISSON).
 // nsys systems in one launch: system k solves kinds[k] from src[k] into dst[k].
+// tabs (may be null): per system the factor table of launch_penta_factor for
+// its kind and lambda; then the workspace holds only the eliminated rhs.
 int launch_penta(const Dims &D, int nsys, const int *kinds, double lambda, const double2 *const *src,
-                 double2 *const *dst, const PentaWS *ws, const double *der, double sx, double sz, hipStream_t s);
+                 double2 *const *dst, const PentaWS *ws, const void *const *tabs, const double *der, double sx,
+                 double sz, hipStream_t s);
+// The matrix part of the D2V or ETA lines for lambda, one row set per
+// (ix, |iz|) group (penta_table_bytes of device memory).
+size_t penta_table_bytes(const Dims &D);
+int launch_penta_factor(const Dims &D, int kind, double lambda, const double *der, void *tab, hipStream_t s);
 
 // D0-type solves with the shared factorisation.  rhs_mode 0: D0 x = src;
 // 1: D0 x = D1 src (src ghost rows filled).  out_mode 0: x into dst;
@@ -101,9 +109,13 @@ int launch_d0solve(const Dims &D, int rhs_mode, int out_mode, const double2 *src
 
 // The exact shift of q[0..nq) (rhs1, rhs2, oldrhs1, oldrhs2): D0 solve at the
 // old shifts, node phase, ghosts at the new shifts, D0 re-weight.
-int launch_exact_shift(const Dims &D, int nq, double2 *const *q, double2 *const *XR, const double *f0rows,
-                       const D0Const &fc, const double *der, const double *y, double dgx, double dgz, double sx0,
-                       double sz0, double sx1, double sz1, hipStream_t s);
+// q[k] is float2 where f32[k] is set.  XR[k]: the chunk entry states.
+int launch_exact_shift(const Dims &D, int nq, void *const *q, const int *f32, double2 *const *XR,
+                       const double *f0rows, const D0Const &fc, const double *der, const double *y, double dgx,
+                       double dgz, double sx0, double sz0, double sx1, double sz1, hipStream_t s);
+// float2 <-> double2 copies of n complex values
+int launch_narrow(const double2 *src, float2 *dst, size_t n, hipStream_t s);
+int launch_widen(const float2 *src, double2 *dst, size_t n, hipStream_t s);
 
 // Ghost rows of nf fields at the shifts (sx, sz).
 int launch_ghosts(const Dims &D, int nf, double2 *const *f, double sx, double sz, hipStream_t s);

@@ -61,8 +61,9 @@ template <typename PR, int MODE, bool SHIFT>
 __global__ void __launch_bounds__(256)
     k_buildrhs(Dims D, RhsArgs a, const double2 *__restrict__ V1, const double2 *__restrict__ V2,
                const double2 *__restrict__ V3, const typename PTraits<PR>::T2 *__restrict__ P0, size_t pstride,
-               double2 *__restrict__ R1, double2 *__restrict__ R2, double2 *__restrict__ O1,
-               double2 *__restrict__ O2, const double *__restrict__ der, const double *__restrict__ y, int rch) {
+               double2 *__restrict__ R1, double2 *__restrict__ R2, typename PTraits<PR>::T2 *__restrict__ O1,
+               typename PTraits<PR>::T2 *__restrict__ O2, const double *__restrict__ der,
+               const double *__restrict__ y, int rch) {
   constexpr bool NONLIN = MODE != 0, PREP = (MODE == 0 || MODE == 3);
   constexpr int Q0 = (MODE == 2 ? 3 : 0), Q1 = (MODE == 1 ? 3 : 6);
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
@@ -194,7 +195,7 @@ __global__ void __launch_bounds__(256)
 }
 
 int launch_buildrhs(const Dims &D, int mixed, const RhsArgs &a, const double2 *V1, const double2 *V2,
-                    const double2 *V3, const void *const P[6], double2 *R1, double2 *R2, double2 *O1, double2 *O2,
+                    const double2 *V3, const void *const P[6], double2 *R1, double2 *R2, void *O1, void *O2,
                     const double *der, const double *y, hipStream_t s) {
   const int rch = 32;
   dim3 grid((D.ncol + 255) / 256, (D.ny + rch - 1) / rch);
@@ -203,7 +204,8 @@ int launch_buildrhs(const Dims &D, int mixed, const RhsArgs &a, const double2 *V
   const bool sh = a.shift != 0;
 #define L(PR, MD, SH)                                                                                            \
   hipLaunchKernelGGL((k_buildrhs<PR, MD, SH>), grid, dim3(256), 0, s, D, a, V1, V2, V3,                          \
-                     (const typename PTraits<PR>::T2 *)P[0], pstride, R1, R2, O1, O2, der, y, rch)
+                     (const typename PTraits<PR>::T2 *)P[0], pstride, R1, R2, (typename PTraits<PR>::T2 *)O1,    \
+                     (typename PTraits<PR>::T2 *)O2, der, y, rch)
 #define M(PR)                                                      \
   switch (mode) {                                                  \
     case 0: if (sh) L(PR, 0, true); else L(PR, 0, false); break;   \
@@ -219,8 +221,9 @@ int launch_buildrhs(const Dims &D, int mixed, const RhsArgs &a, const double2 *V
 
 // shear_shift with the default advection as a pass of its own
 // (src/hst_equations.f90:245-257): rhs, oldrhs times the node phase.
+template <typename OT>
 __global__ void __launch_bounds__(256)
-    k_phase4(Dims D, double2 *R1, double2 *R2, double2 *O1, double2 *O2, const double *__restrict__ y, double dgx,
+    k_phase4(Dims D, double2 *R1, double2 *R2, OT *O1, OT *O2, const double *__restrict__ y, double dgx,
              double dgz) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= D.ncol) return;
@@ -233,10 +236,14 @@ __global__ void __launch_bounds__(256)
   st(O1 + o, ld(O1 + o) * f); st(O2 + o, ld(O2 + o) * f);
 }
 
-int launch_phase4(const Dims &D, double2 *R1, double2 *R2, double2 *O1, double2 *O2, const double *y, double dgx,
-                  double dgz, hipStream_t s) {
+int launch_phase4(const Dims &D, int mixed, double2 *R1, double2 *R2, void *O1, void *O2, const double *y,
+                  double dgx, double dgz, hipStream_t s) {
   dim3 grid((D.ncol + 255) / 256, D.ny);
-  hipLaunchKernelGGL(k_phase4, grid, dim3(256), 0, s, D, R1, R2, O1, O2, y, dgx, dgz);
+  if (mixed)
+    hipLaunchKernelGGL(k_phase4<float2>, grid, dim3(256), 0, s, D, R1, R2, (float2 *)O1, (float2 *)O2, y, dgx, dgz);
+  else
+    hipLaunchKernelGGL(k_phase4<double2>, grid, dim3(256), 0, s, D, R1, R2, (double2 *)O1, (double2 *)O2, y, dgx,
+                       dgz);
   return hipGetLastError();
 }
 
@@ -330,6 +337,7 @@ struct PentaArgs {
   const double2 *src[2];
   double2 *dst[2];
   PentaWS ws[2];
+  const double4 *tab[2];  // factor rows per (ix, |iz|) group, or null: the workspace holds them
 };
 
 // a row of the elimination workspace, as the back substitution reads it
@@ -340,13 +348,90 @@ struct WSRow {
 
 constexpr int PF_FWD = 8, PF_BWD = 4, PF_PENTA = 4;
 
+// The elimination of one cyclic pentadiagonal line with the rows generated
+// from der (line_solve, src/hst_linsolve.f90:282-380, one slab), split into
+// the part that depends only on the matrix (kind, lambda, k^2) and the part
+// that carries the right-hand side.  step() advances the matrix part by row i
+// and returns the multipliers l2, l1, the inverse pivot rp and the row's
+// upper entries t1, t2 and border columns p, q; the record sums accumulate.
+struct PentaFactor {
+  double u11 = 0, u12 = 0, p1 = 0, q1 = 0, u21 = 0, u22 = 0, p2 = 0, q2 = 0;
+  double w1 = 0, w2 = 0, v1 = 0, v2 = 0, sp0 = 0, sq0 = 0, sp1 = 0, sq1 = 0;
+  double ww = 0, vv = 0;  // the weights of the row just stepped
+  template <int K>
+  __device__ __forceinline__ void step(int i, double lam, double ni, double kk, const double *__restrict__ der,
+                                       double &l2, double &l1, double &rp, double &t1, double &t2, double &p,
+                                       double &q) {
+    double a[5];
+#pragma unroll
+    for (int j = 0; j < 5; j++) a[j] = coefk<K>(lam, ni, kk, der, i, j - 2);
+    p = 0.0;
+    q = 0.0;
+    if (i == 0) { p = a[0]; q = a[1]; }
+    if (i == 1) q = a[0];
+    l2 = 0.0;
+    l1 = 0.0;
+    if (i >= 2) {
+      const double l = a[0];
+      a[1] = a[1] - l * u21;
+      a[2] = a[2] - l * u22;
+      p = p - l * p2; q = q - l * q2;
+      l2 = l;
+    }
+    if (i >= 1) {
+      const double l = a[1];
+      a[2] = a[2] - l * u11;
+      a[3] = a[3] - l * u12;
+      p = p - l * p1; q = q - l * q1;
+      l1 = l;
+    }
+    rp = 1.0 / a[2];
+    t1 = a[3] * rp;
+    t2 = a[4] * rp;
+    p = p * rp; q = q * rp;
+    ww = -(u11 * w1 + u22 * w2);
+    if (i == 0) ww = 1.0;
+    vv = -(u11 * v1 + u22 * v2);
+    if (i == 1) vv = 1.0;
+    sp0 = sp0 + ww * p; sq0 = sq0 + ww * q;
+    sp1 = sp1 + vv * p; sq1 = sq1 + vv * q;
+    w2 = w1; w1 = ww; v2 = v1; v1 = vv;
+    u21 = u11; u22 = u12; p2 = p1; q2 = q1;
+    u11 = t1; u12 = t2; p1 = p; q1 = q;
+  }
+};
+
+__device__ __forceinline__ int penta_group(const Dims &D, int ix, int iz) { return ix * (D.nz + 1) + (iz < 0 ? -iz : iz); }
+
+// The matrix part of every (ix, |iz|) group for one kind and lambda:
+// tab[i][g] = (t1, t2, p, q) of row i.  Columns (ix, iz) and (ix, -iz) share
+// k^2 and so every bit of these rows.
+template <int K>
+__global__ void __launch_bounds__(256) k_penta_factor(Dims D, double lam, const double *__restrict__ der,
+                                                      double4 *__restrict__ tab) {
+  const int ng = (D.nx + 1) * (D.nz + 1);
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const int ix = g / (D.nz + 1), iz = g - ix * (D.nz + 1);
+  const double al = D.alfa0 * ix, be = D.beta0 * iz;
+  const double kk = al * al + be * be;
+  PentaFactor f;
+  for (int i = 0; i < D.ny - 2; i++) {
+    double l2, l1, rp, t1, t2, p, q;
+    f.step<K>(i, lam, D.ni, kk, der, l2, l1, rp, t1, t2, p, q);
+    tab[(size_t)i * ng + g] = make_double4(t1, t2, p, q);
+  }
+}
+
 // One cyclic pentadiagonal line per lane (line_solve, src/hst_linsolve.f90:147-483,
 // one slab): forward elimination of rows 0..ny-3 with the rows generated from
-// der, record in registers, 2x2 border system, back substitution.
-template <int K>
+// der, record in registers, 2x2 border system, back substitution.  With a
+// factor table the workspace holds only the eliminated right-hand side.
+template <int K, bool TAB>
 __device__ __forceinline__ void penta_line(const Dims &D, int col, double lam, const double *__restrict__ der,
                                            const double2 *__restrict__ src, double2 *__restrict__ dst,
-                                           const PentaWS &ws, double sx, double sz) {
+                                           const PentaWS &ws, const double4 *__restrict__ tab, double sx,
+                                           double sz) {
   int ix, iz;
   col_ix_iz(D, col, ix, iz);
   const size_t nc = D.ncol;
@@ -358,52 +443,31 @@ __device__ __forceinline__ void penta_line(const Dims &D, int col, double lam, c
   if (ix == 0 && iz == 0 && K == KPOISSON) return;
   const double al = D.alfa0 * ix, be = D.beta0 * iz;
   const double kk = al * al + be * be, ni = D.ni;
-  double u11 = 0, u12 = 0, p1 = 0, q1 = 0, u21 = 0, u22 = 0, p2 = 0, q2 = 0;
-  double w1 = 0, w2 = 0, v1 = 0, v2 = 0, sp0 = 0, sq0 = 0, sp1 = 0, sq1 = 0;
+  PentaFactor f;
   C x1{0, 0}, x2{0, 0}, sx0{0, 0}, sx1{0, 0};
   rows_ascending<PF_PENTA, C>(
       mi, [&](int i) { return ld(src + (size_t)(i + 2) * nc + col); },
       [&](int i, const C &bin) {
-        double a[5];
-#pragma unroll
-        for (int j = 0; j < 5; j++) a[j] = coefk<K>(lam, ni, kk, der, i, j - 2);
-        double p = 0.0, q = 0.0;
-        if (i == 0) { p = a[0]; q = a[1]; }
-        if (i == 1) q = a[0];
+        double l2, l1, rp, t1, t2, p, q;
+        f.step<K>(i, lam, ni, kk, der, l2, l1, rp, t1, t2, p, q);
         C bx = bin;
-        if (i >= 2) {
-          const double l = a[0];
-          a[1] = a[1] - l * u21;
-          a[2] = a[2] - l * u22;
-          bx = bx - l * x2; p = p - l * p2; q = q - l * q2;
-        }
-        if (i >= 1) {
-          const double l = a[1];
-          a[2] = a[2] - l * u11;
-          a[3] = a[3] - l * u12;
-          bx = bx - l * x1; p = p - l * p1; q = q - l * q1;
-        }
-        const double rp = 1.0 / a[2];
-        const double t1 = a[3] * rp, t2 = a[4] * rp;
-        bx = bx * rp; p = p * rp; q = q * rp;
+        if (i >= 2) bx = bx - l2 * x2;
+        if (i >= 1) bx = bx - l1 * x1;
+        bx = bx * rp;
         const size_t o = (size_t)i * nc + col;
-        ws.U1[o] = t1; ws.U2[o] = t2; ws.B1[o] = p; ws.B2[o] = q; st(ws.XR + o, bx);
-        double ww = -(u11 * w1 + u22 * w2);
-        if (i == 0) ww = 1.0;
-        double vv = -(u11 * v1 + u22 * v2);
-        if (i == 1) vv = 1.0;
-        sx0 = sx0 + ww * bx; sp0 = sp0 + ww * p; sq0 = sq0 + ww * q;
-        sx1 = sx1 + vv * bx; sp1 = sp1 + vv * p; sq1 = sq1 + vv * q;
-        w2 = w1; w1 = ww; v2 = v1; v1 = vv;
-        u21 = u11; u22 = u12; x2 = x1; p2 = p1; q2 = q1;
-        u11 = t1; u12 = t2; x1 = bx; p1 = p; q1 = q;
+        if constexpr (!TAB) { ws.U1[o] = t1; ws.U2[o] = t2; ws.B1[o] = p; ws.B2[o] = q; }
+        st(ws.XR + o, bx);
+        sx0 = sx0 + f.ww * bx;
+        sx1 = sx1 + f.vv * bx;
+        x2 = x1;
+        x1 = bx;
       });
   double rr[25];
-  rr[3] = sp0; rr[4] = sq0; rr[5] = w2 * u22 + w1 * u11; rr[6] = w1 * u12;
-  rr[9] = sp1; rr[10] = sq1; rr[11] = v2 * u22 + v1 * u11; rr[12] = v1 * u12;
-  rr[15] = p2 - u21 * p1; rr[16] = q2 - u21 * q1; rr[17] = u22 - u21 * u11; rr[18] = -u21 * u12;
-  rr[21] = p1; rr[22] = q1; rr[23] = u11; rr[24] = u12;
-  const C xm2 = x2 - u21 * x1, xm1 = x1;
+  rr[3] = f.sp0; rr[4] = f.sq0; rr[5] = f.w2 * f.u22 + f.w1 * f.u11; rr[6] = f.w1 * f.u12;
+  rr[9] = f.sp1; rr[10] = f.sq1; rr[11] = f.v2 * f.u22 + f.v1 * f.u11; rr[12] = f.v1 * f.u12;
+  rr[15] = f.p2 - f.u21 * f.p1; rr[16] = f.q2 - f.u21 * f.q1; rr[17] = f.u22 - f.u21 * f.u11; rr[18] = -f.u21 * f.u12;
+  rr[21] = f.p1; rr[22] = f.q1; rr[23] = f.u11; rr[24] = f.u12;
+  const C xm2 = x2 - f.u21 * x1, xm1 = x1;
   const C r25 = ld(src + (size_t)(m - 2 + 2) * nc + col), r27 = ld(src + (size_t)(m - 1 + 2) * nc + col);
   const int ga = m - 2;
   const C ph = cexpi(-(D.alfa0 * ix * sx + D.beta0 * iz * sz));
@@ -419,11 +483,17 @@ __device__ __forceinline__ void penta_line(const Dims &D, int col, double lam, c
   const C cph = conj(ph);
   const C yb1 = cph * b1, yb2 = cph * b2;
   C xk1 = b1, xk2 = b2;
+  const int ng = (D.nx + 1) * (D.nz + 1), g = penta_group(D, ix, iz);
   rows_descending<PF_BWD, WSRow>(
       mi,
       [&](int i) {
         const size_t o = (size_t)i * nc + col;
-        return WSRow{ld(ws.XR + o), ws.B1[o], ws.B2[o], ws.U1[o], ws.U2[o]};
+        if constexpr (TAB) {
+          const double4 t = tab[(size_t)i * ng + g];
+          return WSRow{ld(ws.XR + o), t.z, t.w, t.x, t.y};
+        } else {
+          return WSRow{ld(ws.XR + o), ws.B1[o], ws.B2[o], ws.U1[o], ws.U2[o]};
+        }
       },
       [&](int i, const WSRow &w) {
         const C xk = w.xr - w.b1 * yb1 - w.b2 * yb2 - w.u1 * xk1 - w.u2 * xk2;
@@ -434,30 +504,46 @@ __device__ __forceinline__ void penta_line(const Dims &D, int col, double lam, c
 }
 
 // blockIdx.y selects the system: K0 for the first, K1 for the second
-template <int K0, int K1>
+template <int K0, int K1, bool TAB>
 __global__ void __launch_bounds__(256)
     k_penta(Dims D, PentaArgs pa, double lam, const double *__restrict__ der, double sx, double sz) {
   const int sys = blockIdx.y;
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= D.ncol) return;
-  if (sys == 0) penta_line<K0>(D, col, lam, der, pa.src[0], pa.dst[0], pa.ws[0], sx, sz);
-  else penta_line<K1>(D, col, lam, der, pa.src[1], pa.dst[1], pa.ws[1], sx, sz);
+  if (sys == 0) penta_line<K0, TAB>(D, col, lam, der, pa.src[0], pa.dst[0], pa.ws[0], pa.tab[0], sx, sz);
+  else penta_line<K1, TAB>(D, col, lam, der, pa.src[1], pa.dst[1], pa.ws[1], pa.tab[1], sx, sz);
+}
+
+size_t penta_table_bytes(const Dims &D) { return (size_t)(D.ny - 2) * (D.nx + 1) * (D.nz + 1) * sizeof(double4); }
+
+int launch_penta_factor(const Dims &D, int kind, double lambda, const double *der, void *tab, hipStream_t s) {
+  const int ng = (D.nx + 1) * (D.nz + 1);
+  dim3 grid((ng + 255) / 256);
+  if (kind == KD2V) hipLaunchKernelGGL(k_penta_factor<KD2V>, grid, dim3(256), 0, s, D, lambda, der, (double4 *)tab);
+  else if (kind == KETA) hipLaunchKernelGGL(k_penta_factor<KETA>, grid, dim3(256), 0, s, D, lambda, der, (double4 *)tab);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 int launch_penta(const Dims &D, int nsys, const int *kinds, double lambda, const double2 *const *src,
-                 double2 *const *dst, const PentaWS *ws, const double *der, double sx, double sz, hipStream_t s) {
+                 double2 *const *dst, const PentaWS *ws, const void *const *tabs, const double *der, double sx,
+                 double sz, hipStream_t s) {
   PentaArgs pa;
+  bool tab = tabs != nullptr;
   for (int k = 0; k < nsys; k++) {
     pa.src[k] = src[k];
     pa.dst[k] = dst[k];
     pa.ws[k] = ws[k];
+    pa.tab[k] = tabs ? (const double4 *)tabs[k] : nullptr;
+    if (!pa.tab[k]) tab = false;
   }
   dim3 grid((D.ncol + 255) / 256, nsys);
-#define L(A, B) hipLaunchKernelGGL((k_penta<A, B>), grid, dim3(256), 0, s, D, pa, lambda, der, sx, sz)
-  if (nsys == 2 && kinds[0] == KD2V && kinds[1] == KETA) L(KD2V, KETA);
-  else if (nsys == 1 && kinds[0] == KD2V) L(KD2V, KD2V);
-  else if (nsys == 1 && kinds[0] == KETA) L(KETA, KETA);
-  else if (nsys == 1 && kinds[0] == KPOISSON) L(KPOISSON, KPOISSON);
+#define L(A, B, T) hipLaunchKernelGGL((k_penta<A, B, T>), grid, dim3(256), 0, s, D, pa, lambda, der, sx, sz)
+  if (nsys == 2 && kinds[0] == KD2V && kinds[1] == KETA) {
+    if (tab) L(KD2V, KETA, true); else L(KD2V, KETA, false);
+  } else if (nsys == 1 && kinds[0] == KD2V) L(KD2V, KD2V, false);
+  else if (nsys == 1 && kinds[0] == KETA) L(KETA, KETA, false);
+  else if (nsys == 1 && kinds[0] == KPOISSON) L(KPOISSON, KPOISSON, false);
   else return hipErrorInvalidValue;
 #undef L
   return hipGetLastError();
@@ -521,6 +607,41 @@ __device__ __forceinline__ void d0_forward(const Dims &D, const double *__restri
     push(ld(src + (size_t)(m + 3) * nc + col));
     r27 = stencil(m - 1);
   }
+}
+
+// Forward sweep of D0 x = q as d0_forward<0>, but instead of every row of x
+// only the state entering each chunk of CH_ES rows is kept: CK[2c] = x(a-1),
+// CK[2c+1] = x(a-2) for the chunk c starting at row a = c CH_ES (c >= 1).  The
+// back substitution recomputes a chunk's rows from q and that state, so x is
+// never written out row by row.
+constexpr int CH_ES = 16;
+
+template <typename QT>
+__device__ __forceinline__ void d0_forward_ck(const Dims &D, const double *__restrict__ f0,
+                                              const QT *__restrict__ q, double2 *__restrict__ CK, int col, C &x1,
+                                              C &x2, C &sx0, C &sx1, C &r25, C &r27) {
+  const size_t nc = D.ncol;
+  const int m = D.ny, mi = m - 2;
+  x1 = C{0, 0}; x2 = C{0, 0}; sx0 = C{0, 0}; sx1 = C{0, 0};
+  rows_ascending<PF_FWD, C>(
+      mi, [&](int i) { return ld(q + (size_t)(i + 2) * nc + col); },
+      [&](int i, const C &v) {
+        if (i > 0 && i % CH_ES == 0) {
+          const int c = i / CH_ES;
+          st(CK + (size_t)(2 * c) * nc + col, x1);
+          st(CK + (size_t)(2 * c + 1) * nc + col, x2);
+        }
+        C bx = v;
+        if (i >= 2) bx = bx - F0(i, 0) * x2;
+        if (i >= 1) bx = bx - F0(i, 1) * x1;
+        bx = bx * F0(i, 2);
+        sx0 = sx0 + F0(i, 7) * bx;
+        sx1 = sx1 + F0(i, 8) * bx;
+        x2 = x1;
+        x1 = bx;
+      });
+  r25 = ld(q + (size_t)(m - 2 + 2) * nc + col);
+  r27 = ld(q + (size_t)(m - 1 + 2) * nc + col);
 }
 
 __device__ __forceinline__ void d0_border(const D0Const &fc, C ph, C x1, C x2, C sx0, C sx1, C r25, C r27, C &b1,
@@ -627,24 +748,22 @@ int launch_d0solve(const Dims &D, int rhs_mode, int out_mode, const double2 *src
 // x from the top row down, rows 0 and 1 of x come from the record, so the
 // re-weighted rows are emitted in the same sweep.
 struct QList {
-  double2 *q[4];
-  double2 *xr[4];
+  void *q[4];      // double2, or float2 where f32 is set
+  double2 *xr[4];  // the chunk entry states, 2 (ny - 2 + 15) / 16 rows of ncol
+  int f32[4];
 };
 
-__global__ void __launch_bounds__(256)
-    k_exact_shift(Dims D, D0Const fc, QList ql, const double *__restrict__ f0, const double *__restrict__ der,
-                  const double *__restrict__ y, double dgx, double dgz, double sx0s, double sz0s, double sx1s,
-                  double sz1s) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= D.ncol) return;
-  double2 *__restrict__ q = ql.q[blockIdx.y];
-  double2 *__restrict__ XR = ql.xr[blockIdx.y];
+template <typename QT>
+__device__ __forceinline__ void shift_column(const Dims &D, const D0Const &fc, QT *__restrict__ q,
+                                             double2 *__restrict__ XR, int col, const double *__restrict__ f0,
+                                             const double *__restrict__ der, const double *__restrict__ y, double dgx,
+                                             double dgz, double sx0s, double sz0s, double sx1s, double sz1s) {
   int ix, iz;
   col_ix_iz(D, col, ix, iz);
   const size_t nc = D.ncol;
   const int m = D.ny;
   C x1, x2, sx0, sx1, r25, r27;
-  d0_forward<0>(D, f0, der, q, XR, col, x1, x2, sx0, sx1, r25, r27);
+  d0_forward_ck<QT>(D, f0, q, XR, col, x1, x2, sx0, sx1, r25, r27);
   const C ph0 = cexpi(-(D.alfa0 * ix * sx0s + D.beta0 * iz * sz0s));
   C b1, b2;
   d0_border(fc, ph0, x1, x2, sx0, sx1, r25, r27, b1, b2);
@@ -661,9 +780,7 @@ __global__ void __launch_bounds__(256)
   // window W[k] = x'(i + 1 + k)
   C W0 = xpm2, W1 = xpm1, W2 = xp0 * ph1, W3 = xp1 * ph1;
   C xk1 = b1, xk2 = b2;
-  rows_descending<PF_BWD, C>(
-      m - 2, [&](int i) { return ld(XR + (size_t)i * nc + col); },
-      [&](int i, const C &xr) {
+  auto body = [&](int i, const C &xr) {
         const C xk = xr - F0(i, 5) * yb1 - F0(i, 6) * yb2 - F0(i, 3) * xk1 - F0(i, 4) * xk2;
         xk2 = xk1;
         xk1 = xk;
@@ -677,7 +794,39 @@ __global__ void __launch_bounds__(256)
         acc = acc + DER(der, r, 0, 2) * W3;
         st(q + (size_t)(r + 2) * nc + col, acc);
         W3 = W2; W2 = W1; W1 = W0; W0 = xp;
-      });
+      };
+  // chunks from the top: reload q, recompute x from the chunk's entry state,
+  // back substitute.  The rows of q written here (i + 2) lie above the chunk
+  // being reloaded next.
+  const int mi = m - 2, nch = (mi + CH_ES - 1) / CH_ES;
+  for (int c = nch - 1; c >= 0; c--) {
+    const int a = c * CH_ES;
+    C xs[CH_ES];
+#pragma unroll
+    for (int k = 0; k < CH_ES; k++)
+      if (a + k < mi) xs[k] = ld(q + (size_t)(a + k + 2) * nc + col);
+    C s1{0, 0}, s2{0, 0};
+    if (c > 0) {
+      s1 = ld(XR + (size_t)(2 * c) * nc + col);
+      s2 = ld(XR + (size_t)(2 * c + 1) * nc + col);
+    }
+#pragma unroll
+    for (int k = 0; k < CH_ES; k++) {
+      const int i = a + k;
+      if (i < mi) {
+        C bx = xs[k];
+        if (i >= 2) bx = bx - F0(i, 0) * s2;
+        if (i >= 1) bx = bx - F0(i, 1) * s1;
+        bx = bx * F0(i, 2);
+        xs[k] = bx;
+        s2 = s1;
+        s1 = bx;
+      }
+    }
+#pragma unroll
+    for (int k = CH_ES - 1; k >= 0; k--)
+      if (a + k < mi) body(a + k, xs[k]);
+  }
   // W = x'(0..3); ghosts below: x'(-1) = x'(ny-1) conj(ph1), x'(-2) = x'(ny-2) conj(ph1)
   const C g1 = xpm1 * conj(ph1), g2 = xpm2 * conj(ph1);
   C acc{0, 0};
@@ -696,11 +845,24 @@ __global__ void __launch_bounds__(256)
   st(q + (size_t)(0 + 2) * nc + col, acc);
 }
 
-int launch_exact_shift(const Dims &D, int nq, double2 *const *q, double2 *const *XR, const double *f0rows,
-                       const D0Const &fc, const double *der, const double *y, double dgx, double dgz, double sx0,
-                       double sz0, double sx1, double sz1, hipStream_t s) {
+__global__ void __launch_bounds__(256)
+    k_exact_shift(Dims D, D0Const fc, QList ql, const double *__restrict__ f0, const double *__restrict__ der,
+                  const double *__restrict__ y, double dgx, double dgz, double sx0s, double sz0s, double sx1s,
+                  double sz1s) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= D.ncol) return;
+  const int k = blockIdx.y;
+  if (ql.f32[k])
+    shift_column<float2>(D, fc, (float2 *)ql.q[k], ql.xr[k], col, f0, der, y, dgx, dgz, sx0s, sz0s, sx1s, sz1s);
+  else
+    shift_column<double2>(D, fc, (double2 *)ql.q[k], ql.xr[k], col, f0, der, y, dgx, dgz, sx0s, sz0s, sx1s, sz1s);
+}
+
+int launch_exact_shift(const Dims &D, int nq, void *const *q, const int *f32, double2 *const *XR,
+                       const double *f0rows, const D0Const &fc, const double *der, const double *y, double dgx,
+                       double dgz, double sx0, double sz0, double sx1, double sz1, hipStream_t s) {
   QList ql;
-  for (int k = 0; k < nq; k++) { ql.q[k] = q[k]; ql.xr[k] = XR[k]; }
+  for (int k = 0; k < nq; k++) { ql.q[k] = q[k]; ql.xr[k] = XR[k]; ql.f32[k] = f32[k]; }
   dim3 grid((D.ncol + 255) / 256, nq);
   hipLaunchKernelGGL(k_exact_shift, grid, dim3(256), 0, s, D, fc, ql, f0rows, der, y, dgx, dgz, sx0, sz0, sx1, sz1);
   return hipGetLastError();
@@ -845,6 +1007,26 @@ int launch_product_to_field(const Dims &D, int mixed, const void *P, double2 *ds
     hipLaunchKernelGGL(k_product_to_field<float>, grid, dim3(256), 0, s, (const float2 *)P, dst, n);
   else
     hipLaunchKernelGGL(k_product_to_field<double>, grid, dim3(256), 0, s, (const double2 *)P, dst, n);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) k_narrow(const double2 *__restrict__ src, float2 *__restrict__ dst, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = make_float2((float)src[i].x, (float)src[i].y);
+}
+
+__global__ void __launch_bounds__(256) k_widen(const float2 *__restrict__ src, double2 *__restrict__ dst, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = make_double2((double)src[i].x, (double)src[i].y);
+}
+
+int launch_narrow(const double2 *src, float2 *dst, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_narrow, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+
+int launch_widen(const float2 *src, double2 *dst, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_widen, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
   return hipGetLastError();
 }
 

@@ -165,9 +165,11 @@ def test_one_step_wide_grids(O, n, ny):
 
 
 @pytest.mark.parametrize("exact", [False, True])
-@pytest.mark.parametrize("s2", [False, True])
-def test_rhs_and_shift(O, exact, s2):
-    """buildrhs_prepare, buildrhs(1), buildrhs(2) and shear_shift vs the oracle."""
+@pytest.mark.parametrize("s2,precision,tol", [(False, hst.FP64, 1e-12), (True, hst.FP64, 1e-12),
+                                              (True, hst.MIXED, 2e-5)])
+def test_rhs_and_shift(O, exact, s2, precision, tol):
+    """buildrhs_prepare, buildrhs(1), buildrhs(2) and shear_shift vs the oracle
+    (the mixed form: FP32 products and oldrhs, FP64 rhs and shift solves)."""
     d = read_input(deck_path("small"))
     d = dataclasses.replace(d, exact_shift=exact)
     if s2:
@@ -192,21 +194,23 @@ def test_rhs_and_shift(O, exact, s2):
     dt_sub = 2.0 / ode[0] * dt
     o_lib.hsto_shear_shift(o.h, dt_sub, t)
     post = [o.field(f).copy() for f in (4, 5, 6, 7)]
-    with GPU(d) as g:
+    with GPU(d, precision) as g:
         g.set_V(V0)
         g.upload(hst.OLD1, old[0])
         g.upload(hst.OLD2, old[1])
+        if precision == hst.MIXED:  # oldrhs is FP32 in the mixed form
+            assert 0 < rel(g.download(hst.OLD1), old[0]) < 1e-7
         g.nonlinear()
         g.buildrhs_prepare(ode, dt, t)
         g.buildrhs(ode, 1)
         g.buildrhs(ode, 2)
         got = [g.download(f) for f in (4, 5, 6, 7)]
         for a, b in zip(got, pre):
-            assert rel(a[..., 2:-2], b[..., 2:-2]) <= 1e-12
+            assert rel(a[..., 2:-2], b[..., 2:-2]) <= tol
         g.shear_shift(dt_sub, t)
         got = [g.download(f) for f in (4, 5, 6, 7)]
         for a, b in zip(got, post):
-            assert rel(a[..., 2:-2], b[..., 2:-2]) <= 1e-12
+            assert rel(a[..., 2:-2], b[..., 2:-2]) <= tol
 
 
 def test_linsolve(O):
