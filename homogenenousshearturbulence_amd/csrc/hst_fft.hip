@@ -77,7 +77,11 @@ struct ZCfg {
   static constexpr int T = fft_threads(N);
   static constexpr int LDS_MAX = 100 * 1024;
   static constexpr int G0 = (sizeof(R) == 8 ? 8 : 16);  // 128-byte transposed segments
-  static constexpr int g_lds(int g) { return (g > 1 && g * (N + 1) * (int)sizeof(Cx<R>) > LDS_MAX) ? g_lds(g / 2) : g; }
+  // LDS line stride: lanes of the transposed accesses walk the lines (stride LS)
+  // and the points (stride 1); LS = 16 dwords mod 64 spreads them over all banks
+  static constexpr int DW = (int)sizeof(Cx<R>) / 4, PER = 64 / DW;
+  static constexpr int LS = N + (((16 / DW) - N % PER) % PER + PER) % PER;
+  static constexpr int g_lds(int g) { return (g > 1 && g * LS * (int)sizeof(Cx<R>) > LDS_MAX) ? g_lds(g / 2) : g; }
   static constexpr int g_thr(int g) { return (g > 1 && g * T > 1024) ? g_thr(g / 2) : g; }
   static constexpr int G = g_thr(g_lds(G0));
   static constexpr int NT = G * T;
@@ -119,7 +123,7 @@ template <typename R, int NZD>
 __global__ void __launch_bounds__((ZCfg<R, NZD>::NT))
     k_zift(const double2 *__restrict__ V1, const double2 *__restrict__ V2, const double2 *__restrict__ V3,
            Cx<R> *__restrict__ Hv, Dims D, int row0, int nrows, const Cx<R> *__restrict__ tw) {
-  constexpr int G = ZCfg<R, NZD>::G, T = ZCfg<R, NZD>::T, LS = NZD + 1;
+  constexpr int G = ZCfg<R, NZD>::G, T = ZCfg<R, NZD>::T, LS = ZCfg<R, NZD>::LS;
   __shared__ Cx<R> lds[G * LS];
   __shared__ Cx<R> twl[NZD];  // the twiddles: lanes of a stage read scattered entries
   for (int k = threadIdx.x; k < NZD; k += blockDim.x) twl[k] = tw[k];
@@ -155,7 +159,7 @@ template <typename R, int NZD>
 __global__ void __launch_bounds__((ZCfg<R, NZD>::NT))
     k_zfft(const Cx<R> *__restrict__ Hp, Cx<R> *__restrict__ P0, size_t pstride, Dims D, int row0, int nrows,
            const Cx<R> *__restrict__ tw) {
-  constexpr int G = ZCfg<R, NZD>::G, T = ZCfg<R, NZD>::T, LS = NZD + 1;
+  constexpr int G = ZCfg<R, NZD>::G, T = ZCfg<R, NZD>::T, LS = ZCfg<R, NZD>::LS;
   __shared__ Cx<R> lds[G * LS];
   __shared__ Cx<R> twl[NZD];
   for (int k = threadIdx.x; k < NZD; k += blockDim.x) twl[k] = tw[k];
@@ -359,8 +363,13 @@ __device__ __forceinline__ void xl_c2r_load(Cx<R> (&dst)[XL<M>::E], const Cx<R> 
   }
 }
 
+// two wavefronts per SIMD at least: the FP64 lines of M <= 512 fit in 256
+// VGPRs with a few bytes of spill instead of taking one wavefront per SIMD
+template <typename R, int M>
+constexpr int xline_min_waves() { return (sizeof(R) == 8 && M >= 1024) ? 1 : 2; }
+
 template <typename R, int M, int MODE>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, (xline_min_waves<R, M>()))
     k_xline(const Cx<R> *__restrict__ Hv, Cx<R> *__restrict__ Hp, Dims D, int nrows, const Cx<R> *__restrict__ twx,
             const Cx<R> *__restrict__ twh, const double *__restrict__ dyl, double *__restrict__ cfl_part,
             int want_cfl, int row0) {

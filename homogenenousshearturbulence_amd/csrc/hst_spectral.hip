@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(256)
 int launch_buildrhs(const Dims &D, int mixed, const RhsArgs &a, const double2 *V1, const double2 *V2,
                     const double2 *V3, const void *const P[6], double2 *R1, double2 *R2, void *O1, void *O2,
                     const double *der, const double *y, hipStream_t s) {
-  const int rch = 32;
+  const int rch = 64;  // rows per lane: the 5-row windows re-read 4 rows per chunk
   dim3 grid((D.ncol + 255) / 256, (D.ny + rch - 1) / rch);
   const size_t pstride = (size_t)D.NYg * D.ncol;
   const int mode = a.nonlinear;  // 0 prepare, 1/2 one group, 3 all
