@@ -109,7 +109,7 @@ int launch_d0solve(const Dims &D, int rhs_mode, int out_mode, const double2 *src
 
 // The exact shift of q[0..nq) (rhs1, rhs2, oldrhs1, oldrhs2): D0 solve at the
 // old shifts, node phase, ghosts at the new shifts, D0 re-weight.
-// q[k] is float2 where f32[k] is set.  XR[k]: the chunk entry states.
+// q[k] is float2 where f32[k] is set.  XR[k]: workspace of ny - 2 rows.
 int launch_exact_shift(const Dims &D, int nq, void *const *q, const int *f32, double2 *const *XR,
                        const double *f0rows, const D0Const &fc, const double *der, const double *y, double dgx,
                        double dgz, double sx0, double sz0, double sx1, double sz1, hipStream_t s);

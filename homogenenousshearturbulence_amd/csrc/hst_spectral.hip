@@ -557,9 +557,9 @@ int launch_penta(const Dims &D, int nsys, const int *kinds, double lambda, const
 // Forward sweep of D0 x = r for one lane, XR stored; r = src (RHS 0) or the
 // D1 stencil of src (RHS 1, ghost rows filled).  Also returns the border
 // rows' right-hand sides r(ny-2), r(ny-1).
-template <int RHS>
+template <int RHS, typename ST = double2>
 __device__ __forceinline__ void d0_forward(const Dims &D, const double *__restrict__ f0,
-                                           const double *__restrict__ der, const double2 *__restrict__ src,
+                                           const double *__restrict__ der, const ST *__restrict__ src,
                                            double2 *__restrict__ XR, int col, C &x1, C &x2, C &sx0, C &sx1, C &r25,
                                            C &r27) {
   const size_t nc = D.ncol;
@@ -607,41 +607,6 @@ __device__ __forceinline__ void d0_forward(const Dims &D, const double *__restri
     push(ld(src + (size_t)(m + 3) * nc + col));
     r27 = stencil(m - 1);
   }
-}
-
-// Forward sweep of D0 x = q as d0_forward<0>, but instead of every row of x
-// only the state entering each chunk of CH_ES rows is kept: CK[2c] = x(a-1),
-// CK[2c+1] = x(a-2) for the chunk c starting at row a = c CH_ES (c >= 1).  The
-// back substitution recomputes a chunk's rows from q and that state, so x is
-// never written out row by row.
-constexpr int CH_ES = 16;
-
-template <typename QT>
-__device__ __forceinline__ void d0_forward_ck(const Dims &D, const double *__restrict__ f0,
-                                              const QT *__restrict__ q, double2 *__restrict__ CK, int col, C &x1,
-                                              C &x2, C &sx0, C &sx1, C &r25, C &r27) {
-  const size_t nc = D.ncol;
-  const int m = D.ny, mi = m - 2;
-  x1 = C{0, 0}; x2 = C{0, 0}; sx0 = C{0, 0}; sx1 = C{0, 0};
-  rows_ascending<PF_FWD, C>(
-      mi, [&](int i) { return ld(q + (size_t)(i + 2) * nc + col); },
-      [&](int i, const C &v) {
-        if (i > 0 && i % CH_ES == 0) {
-          const int c = i / CH_ES;
-          st(CK + (size_t)(2 * c) * nc + col, x1);
-          st(CK + (size_t)(2 * c + 1) * nc + col, x2);
-        }
-        C bx = v;
-        if (i >= 2) bx = bx - F0(i, 0) * x2;
-        if (i >= 1) bx = bx - F0(i, 1) * x1;
-        bx = bx * F0(i, 2);
-        sx0 = sx0 + F0(i, 7) * bx;
-        sx1 = sx1 + F0(i, 8) * bx;
-        x2 = x1;
-        x1 = bx;
-      });
-  r25 = ld(q + (size_t)(m - 2 + 2) * nc + col);
-  r27 = ld(q + (size_t)(m - 1 + 2) * nc + col);
 }
 
 __device__ __forceinline__ void d0_border(const D0Const &fc, C ph, C x1, C x2, C sx0, C sx1, C r25, C r27, C &b1,
@@ -749,7 +714,7 @@ int launch_d0solve(const Dims &D, int rhs_mode, int out_mode, const double2 *src
 // re-weighted rows are emitted in the same sweep.
 struct QList {
   void *q[4];      // double2, or float2 where f32 is set
-  double2 *xr[4];  // the chunk entry states, 2 (ny - 2 + 15) / 16 rows of ncol
+  double2 *xr[4];  // the D0 solution of each array, ny - 2 rows of ncol
   int f32[4];
 };
 
@@ -763,7 +728,7 @@ __device__ __forceinline__ void shift_column(const Dims &D, const D0Const &fc, Q
   const size_t nc = D.ncol;
   const int m = D.ny;
   C x1, x2, sx0, sx1, r25, r27;
-  d0_forward_ck<QT>(D, f0, q, XR, col, x1, x2, sx0, sx1, r25, r27);
+  d0_forward<0, QT>(D, f0, der, q, XR, col, x1, x2, sx0, sx1, r25, r27);
   const C ph0 = cexpi(-(D.alfa0 * ix * sx0s + D.beta0 * iz * sz0s));
   C b1, b2;
   d0_border(fc, ph0, x1, x2, sx0, sx1, r25, r27, b1, b2);
@@ -795,38 +760,7 @@ __device__ __forceinline__ void shift_column(const Dims &D, const D0Const &fc, Q
         st(q + (size_t)(r + 2) * nc + col, acc);
         W3 = W2; W2 = W1; W1 = W0; W0 = xp;
       };
-  // chunks from the top: reload q, recompute x from the chunk's entry state,
-  // back substitute.  The rows of q written here (i + 2) lie above the chunk
-  // being reloaded next.
-  const int mi = m - 2, nch = (mi + CH_ES - 1) / CH_ES;
-  for (int c = nch - 1; c >= 0; c--) {
-    const int a = c * CH_ES;
-    C xs[CH_ES];
-#pragma unroll
-    for (int k = 0; k < CH_ES; k++)
-      if (a + k < mi) xs[k] = ld(q + (size_t)(a + k + 2) * nc + col);
-    C s1{0, 0}, s2{0, 0};
-    if (c > 0) {
-      s1 = ld(XR + (size_t)(2 * c) * nc + col);
-      s2 = ld(XR + (size_t)(2 * c + 1) * nc + col);
-    }
-#pragma unroll
-    for (int k = 0; k < CH_ES; k++) {
-      const int i = a + k;
-      if (i < mi) {
-        C bx = xs[k];
-        if (i >= 2) bx = bx - F0(i, 0) * s2;
-        if (i >= 1) bx = bx - F0(i, 1) * s1;
-        bx = bx * F0(i, 2);
-        xs[k] = bx;
-        s2 = s1;
-        s1 = bx;
-      }
-    }
-#pragma unroll
-    for (int k = CH_ES - 1; k >= 0; k--)
-      if (a + k < mi) body(a + k, xs[k]);
-  }
+  rows_descending<PF_BWD, C>(m - 2, [&](int i) { return ld(XR + (size_t)i * nc + col); }, body);
   // W = x'(0..3); ghosts below: x'(-1) = x'(ny-1) conj(ph1), x'(-2) = x'(ny-2) conj(ph1)
   const C g1 = xpm1 * conj(ph1), g2 = xpm2 * conj(ph1);
   C acc{0, 0};
