@@ -68,6 +68,13 @@ struct State {
   bool ptab_off = false;
   void *tables = nullptr;
   hipStream_t stream = nullptr;
+  // the oldrhs half of the exact shift runs on a second stream, beside the
+  // solves and the next substep's transforms, and is joined before the next
+  // buildrhs (or before any other entry point)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool side_pending = false;
+  double2 *XRo[2] = {nullptr, nullptr};
   int64_t bytes_held = 0;
   bool prof = false;
   std::vector<Pending> pending;
@@ -110,6 +117,7 @@ static hipEvent_t take_event() {
 static void flush_profile() {
   if (S.pending.empty()) return;
   (void)hipStreamSynchronize(S.stream);
+  if (S.side) (void)hipStreamSynchronize(S.side);
   for (auto &p : S.pending) {
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, p.a, p.b);
@@ -124,23 +132,28 @@ static void flush_profile() {
   S.pending.clear();
 }
 
-// Runs one launcher; with timing on, brackets it with events on the stream.
+// Runs one launcher; with timing on, brackets it with events on its stream.
 template <class F>
-static int run(const char *name, double bytes, F launch) {
+static int run_on(hipStream_t st, const char *name, double bytes, F launch) {
   hipEvent_t a = nullptr, b = nullptr;
   if (S.prof) {
     a = take_event();
     b = take_event();
-    (void)hipEventRecord(a, S.stream);
+    (void)hipEventRecord(a, st);
   }
   int rc = launch();
   if (rc) return fail(std::string("launch of ") + name + " failed: " + hipGetErrorString((hipError_t)rc), rc);
   if (S.prof) {
-    (void)hipEventRecord(b, S.stream);
+    (void)hipEventRecord(b, st);
     S.pending.push_back(Pending{name, a, b, bytes});
     if (S.pending.size() > 4096) flush_profile();
   }
   return 0;
+}
+
+template <class F>
+static int run(const char *name, double bytes, F launch) {
+  return run_on(S.stream, name, bytes, launch);
 }
 
 int profiled(const char *name, double bytes, const std::function<int()> &launch) {
@@ -285,6 +298,34 @@ static int exact_shift(double dt_sub, double time) {
   });
 }
 
+// The exact shift inside a time step: rhs on the main stream (the solves read
+// it next), oldrhs on the side stream, joined before the next buildrhs.
+static int exact_shift_split(double dt_sub, double time) {
+  const double dgx = S.deck.S * dt_sub, dgz = s2_integral(time, time + dt_sub);
+  double sx0, sz0, sx1, sz1;
+  shear_shifts(time, sx0, sz0);
+  shear_shifts(time + dt_sub, sx1, sz1);
+  void *qr[2] = {S.F[HST_RHS1], S.F[HST_RHS2]}, *qo[2] = {S.F[HST_OLD1], S.F[HST_OLD2]};
+  const int fr[2] = {0, 0}, fo[2] = {S.mixed, S.mixed};
+  double2 *xr[2] = {S.XR[0], S.XR[1]};
+  const double os = S.mixed ? 8.0 : 16.0;
+  HIPCHK(hipEventRecord(S.ev_fork, S.stream));
+  int rc = run("exact_shift", 2.0 * (2.0 * 16.0 + 2.0 * 16.0) / 16.0 * bytes_field(), [&] {
+    return launch_exact_shift(S.D, 2, qr, fr, xr, S.d_f0, S.fc, S.d_der, S.d_y, dgx, dgz, sx0, sz0, sx1, sz1,
+                              S.stream);
+  });
+  if (rc) return rc;
+  HIPCHK(hipStreamWaitEvent(S.side, S.ev_fork, 0));
+  rc = run_on(S.side, "exact_shift_old", 2.0 * (2.0 * os + 2.0 * 16.0) / 16.0 * bytes_field(), [&] {
+    return launch_exact_shift(S.D, 2, qo, fo, S.XRo, S.d_f0, S.fc, S.d_der, S.d_y, dgx, dgz, sx0, sz0, sx1, sz1,
+                              S.side);
+  });
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(S.ev_join, S.side));
+  S.side_pending = true;
+  return 0;
+}
+
 static int shear_shift(double dt_sub, double time) {
   if (S.deck.exact_shift) return exact_shift(dt_sub, time);
   const double dgx = S.deck.S * dt_sub, dgz = s2_integral(time, time + dt_sub);
@@ -356,9 +397,22 @@ static int linsolve(double lambda, double time) {
   return ghosts(2, g13, sx, sz);
 }
 
-static int check_live() {
+// the main stream waits for the oldrhs shift of the side stream
+static int join_side() {
+  if (!S.side_pending) return 0;
+  S.side_pending = false;
+  HIPCHK(hipStreamWaitEvent(S.stream, S.ev_join, 0));
+  return 0;
+}
+
+static int check_live_nojoin() {
   if (!S.live) return fail("hst_gpu_init has not been called");
   return 0;
+}
+
+static int check_live() {
+  if (int rc = check_live_nojoin()) return rc;
+  return join_side();
 }
 
 static int check_field(int f) {
@@ -375,6 +429,7 @@ static int check_field64(int f) {
 
 static void release() {
   if (S.stream) (void)hipStreamSynchronize(S.stream);
+  if (S.side) (void)hipStreamSynchronize(S.side);
   for (int i = 1; i <= 8; i++) if (S.F[i]) (void)hipFree(S.F[i]);
   void *ptrs[] = {S.P, S.Hv, S.Hp, S.ws, S.d_der, S.d_y, S.d_dyl, S.d_f0, S.d_part};
   for (void *p : ptrs) if (p) (void)hipFree(p);
@@ -384,6 +439,10 @@ static void release() {
   for (auto &p : S.pending) { S.pool.push_back(p.a); S.pool.push_back(p.b); }
   for (auto e : S.pool) (void)hipEventDestroy(e);
   if (S.stream) (void)hipStreamDestroy(S.stream);
+  if (S.side) (void)hipStreamDestroy(S.side);
+  if (S.ev_fork) (void)hipEventDestroy(S.ev_fork);
+  if (S.ev_join) (void)hipEventDestroy(S.ev_join);
+  for (double2 *p : S.XRo) if (p) (void)hipFree(p);
   S = State();
 }
 
@@ -431,6 +490,9 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
   S.fc = d0_const(S.f0);
 
   HIPCHK(hipStreamCreate(&S.stream));
+  HIPCHK(hipStreamCreate(&S.side));
+  HIPCHK(hipEventCreateWithFlags(&S.ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&S.ev_join, hipEventDisableTiming));
   const size_t nf = (size_t)D.NYg * D.ncol;
   const size_t cs = S.mixed ? sizeof(float2) : sizeof(double2);
   int64_t held = 0;
@@ -459,6 +521,12 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
     S.pws[k].XR = (double2 *)(b + rows * 32);
   }
   for (int k = 0; k < 4; k++) S.XR[k] = (double2 *)(w + k * rows * 16);
+  if (deck->exact_shift) {
+    for (int k = 0; k < 2; k++) {
+      HIPCHK(hipMalloc(&S.XRo[k], rows * sizeof(double2)));
+      held += rows * sizeof(double2);
+    }
+  }
   // transform chunk: rows of the physical-space buffers Hv (3 fields) and Hp (6)
   const size_t per_row = 9 * (size_t)D.nzd * D.NXp * cs;
   size_t free_b = 0, total_b = 0;
@@ -625,7 +693,7 @@ int hst_gpu_linsolve(double lambda, double time) {
 }
 
 int hst_gpu_timestep(double deltat, double *time, double *cfl) {
-  if (int rc = check_live()) return rc;
+  if (int rc = check_live_nojoin()) return rc;
   if (!time) return fail("time is NULL");
   double t = *time, c = 0.0;
   for (int i = 0; i < 3; i++) {
@@ -640,9 +708,10 @@ int hst_gpu_timestep(double deltat, double *time, double *cfl) {
     const double dt_sub = 2.0 / ode[0] * deltat;
     const bool fuse_shift = !S.deck.exact_shift;
     const double dgx = S.deck.S * dt_sub, dgz = s2_integral(t, t + dt_sub);
+    if ((rc = join_side())) return rc;  // the previous substep's oldrhs shift
     rc = buildrhs(ode, deltat, t, S.deck.linear ? 0 : 3, fuse_shift, dgx, dgz);
     if (rc) return rc;
-    if (S.deck.exact_shift && (rc = exact_shift(dt_sub, t))) return rc;
+    if (S.deck.exact_shift && (rc = exact_shift_split(dt_sub, t))) return rc;
     t = t + dt_sub;
     if ((rc = linsolve(ode[0] / deltat, t))) return rc;
   }

@@ -480,13 +480,26 @@ __global__ void __launch_bounds__(64)
 
 // Ordered sum of nblk partial rows per parameter, then (update) the Adam step
 // or (!update) the gradient into grad; slot NP: the summed squared error.
-__global__ void __launch_bounds__(256)
+// A block holds 64 parameters (one per lane of a wavefront, so every row read
+// is 256 contiguous bytes) and AR row groups (one per wavefront); the group
+// sums are added in a fixed order, so the result does not depend on timing.
+constexpr int AR = 8;
+__global__ void __launch_bounds__(64 * AR)
     k_adam(float *__restrict__ par, float *__restrict__ m1, float *__restrict__ m2, const float *__restrict__ part,
            int nblk, int update, float lr, float c1, float c2, float *__restrict__ grad, double *__restrict__ sse) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p > NP) return;
+  __shared__ double red[AR][64];
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int p = blockIdx.x * 64 + l;
   double g = 0.0;
-  for (int b = 0; b < nblk; b++) g += (double)part[(size_t)b * NPP + p];
+  if (p <= NP) {
+#pragma unroll 8
+    for (int b = w; b < nblk; b += AR) g += (double)part[(size_t)b * NPP + p];
+  }
+  red[w][l] = g;
+  __syncthreads();
+  if (w != 0 || p > NP) return;
+  g = 0.0;
+  for (int k = 0; k < AR; k++) g += red[k][l];
   if (p == NP) {
     *sse = g;
     return;
@@ -512,13 +525,15 @@ struct InferIO {
   int mode;
 };
 
+constexpr int INFER_TILES = 32;  // 16-sample tiles per block: the weight staging is paid once per 512 samples
+
 __global__ void __launch_bounds__(64) k_mlp_infer(const float *__restrict__ par, Norm nm, InferIO io) {
   __shared__ WLds W;
   const int l = threadIdx.x, j = l & 15, q = l >> 4;
   stage_weights(par, W, l);
   __syncthreads();
-  for (int jt = 0; jt < 4; jt++) {
-    const int64_t s = (int64_t)blockIdx.x * 64 + jt * 16 + j;
+  for (int jt = 0; jt < INFER_TILES; jt++) {
+    const int64_t s = ((int64_t)blockIdx.x * INFER_TILES + jt) * 16 + j;
     const bool valid = s < io.n;
     bf8 x;
     if (io.mode == 0) x = input_operand(q, valid, nm, [&](int k) { return io.X[s * 9 + k]; });
@@ -742,7 +757,8 @@ int stress(hst_closure *c, const double2 *const U[3], double sx, double sz, hipS
   io.fs = c->fs();
   io.n = (int64_t)c->fs();
   io.mode = 1;
-  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((io.n + 63) / 64)), dim3(64), 0, st, c->par, make_norm(c->norm), io);
+  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((io.n + 16 * INFER_TILES - 1) / (16 * INFER_TILES))), dim3(64), 0,
+                     st, c->par, make_norm(c->norm), io);
   Cx<double> *Hp = c->Hc;  // the z-inverse output is free again
   switch (D.nxd) {
 #define XC(n) case n: run_cxr2c<n>(D, c->phys + 9 * c->fs(), Hp, c->twx, c->twh, st); break;
@@ -1115,7 +1131,7 @@ int hst_closure_grad(hst_closure *c, int64_t first, int64_t n, float *grad, doub
   const int nblk = (int)((n + SPB - 1) / SPB);
   hipLaunchKernelGGL(k_mlp_train<true>, dim3(nblk), dim3(64), 0, st, c->par, make_norm(c->norm), ta, c->part,
                      c->lossp);
-  hipLaunchKernelGGL(k_adam, dim3((NP + 1 + 255) / 256), dim3(256), 0, st, c->par, c->m1, c->m2, c->part, nblk, 0,
+  hipLaunchKernelGGL(k_adam, dim3((NP + 1 + 63) / 64), dim3(64 * AR), 0, st, c->par, c->m1, c->m2, c->part, nblk, 0,
                      0.0f, 0.0f, 0.0f, c->gradd, c->sse);
   CHK(hipGetLastError());
   double sse = 0.0;
@@ -1176,7 +1192,7 @@ int hst_closure_train(hst_closure *c, int epochs, int batch, double lr, uint64_t
       const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, t))), c2 = (float)(1.0 / (1.0 - std::pow(0.999, t)));
       TrainArgs ta{c->X, c->Yd, batch, b * batch, mul, add, n_train, (float)(2.0 / ((double)batch * 6.0))};
       hipLaunchKernelGGL(k_mlp_train<true>, dim3(nblk), dim3(64), 0, st, c->par, nm, ta, c->part, c->lossp);
-      hipLaunchKernelGGL(k_adam, dim3((NP + 1 + 255) / 256), dim3(256), 0, st, c->par, c->m1, c->m2, c->part, nblk,
+      hipLaunchKernelGGL(k_adam, dim3((NP + 1 + 63) / 64), dim3(64 * AR), 0, st, c->par, c->m1, c->m2, c->part, nblk,
                          1, (float)lr, c1, c2, c->gradd, dsse + b);
     }
     hipError_t err = hipGetLastError();
@@ -1218,7 +1234,8 @@ int hst_closure_predict(hst_closure *c, int64_t n, const float *x, float *y) {
   InferIO io{dx, dy, nullptr, nullptr, 0, n, 0};
   hipError_t e = hipMemcpyAsync(dx, x, (size_t)n * 9 * sizeof(float), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, c->par, make_norm(c->norm), io);
+    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 16 * INFER_TILES - 1) / (16 * INFER_TILES))), dim3(64), 0,
+                       st, c->par, make_norm(c->norm), io);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpyAsync(y, dy, (size_t)n * 6 * sizeof(float), hipMemcpyDeviceToHost, st);

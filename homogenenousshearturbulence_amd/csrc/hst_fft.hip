@@ -365,8 +365,11 @@ __device__ __forceinline__ void xl_c2r_load(Cx<R> (&dst)[XL<M>::E], const Cx<R> 
 
 // two wavefronts per SIMD at least: the FP64 lines of M <= 512 fit in 256
 // VGPRs with a few bytes of spill instead of taking one wavefront per SIMD
+#ifndef XLINE_F32_WAVES
+#define XLINE_F32_WAVES 3
+#endif
 template <typename R, int M>
-constexpr int xline_min_waves() { return (sizeof(R) == 8 && M >= 1024) ? 1 : 2; }
+constexpr int xline_min_waves() { return sizeof(R) == 8 ? (M >= 1024 ? 1 : 2) : (M >= 1024 ? 2 : XLINE_F32_WAVES); }
 
 template <typename R, int M, int MODE>
 __global__ void __launch_bounds__(64, (xline_min_waves<R, M>()))
