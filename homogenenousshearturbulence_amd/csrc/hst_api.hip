@@ -75,6 +75,7 @@ struct State {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool side_pending = false;
   double2 *XRo[2] = {nullptr, nullptr};
+  double2 *PX = nullptr, *PZ = nullptr;  // node phases of the exact shift
   int64_t bytes_held = 0;
   bool prof = false;
   std::vector<Pending> pending;
@@ -288,12 +289,16 @@ static int exact_shift(double dt_sub, double time) {
   double sx0, sz0, sx1, sz1;
   shear_shifts(time, sx0, sz0);
   shear_shifts(time + dt_sub, sx1, sz1);
+  if (int rc = run("shift_phases", 0.0, [&] {
+        return launch_shift_phases(S.D, S.d_y, dgx, dgz, S.PX, S.PZ, S.stream);
+      }))
+    return rc;
   void *q[4] = {S.F[HST_RHS1], S.F[HST_RHS2], S.F[HST_OLD1], S.F[HST_OLD2]};
   const int f32[4] = {0, 0, S.mixed, S.mixed};
   // per array: q in, the solution x out and back in (FP64), q out
   const double qb = 2.0 * 2.0 * 16.0 + 2.0 * 2.0 * (S.mixed ? 8.0 : 16.0) + 4.0 * 2.0 * 16.0;
   return run("exact_shift", qb / 16.0 * bytes_field(), [&] {
-    return launch_exact_shift(S.D, 4, q, f32, S.XR, S.d_f0, S.fc, S.d_der, S.d_y, dgx, dgz, sx0, sz0, sx1, sz1,
+    return launch_exact_shift(S.D, 4, q, f32, S.XR, S.d_f0, S.fc, S.d_der, S.PX, S.PZ, sx0, sz0, sx1, sz1,
                               S.stream);
   });
 }
@@ -309,15 +314,19 @@ static int exact_shift_split(double dt_sub, double time) {
   const int fr[2] = {0, 0}, fo[2] = {S.mixed, S.mixed};
   double2 *xr[2] = {S.XR[0], S.XR[1]};
   const double os = S.mixed ? 8.0 : 16.0;
+  if (int rc = run("shift_phases", 0.0, [&] {
+        return launch_shift_phases(S.D, S.d_y, dgx, dgz, S.PX, S.PZ, S.stream);
+      }))
+    return rc;
   HIPCHK(hipEventRecord(S.ev_fork, S.stream));
   int rc = run("exact_shift", 2.0 * (2.0 * 16.0 + 2.0 * 16.0) / 16.0 * bytes_field(), [&] {
-    return launch_exact_shift(S.D, 2, qr, fr, xr, S.d_f0, S.fc, S.d_der, S.d_y, dgx, dgz, sx0, sz0, sx1, sz1,
+    return launch_exact_shift(S.D, 2, qr, fr, xr, S.d_f0, S.fc, S.d_der, S.PX, S.PZ, sx0, sz0, sx1, sz1,
                               S.stream);
   });
   if (rc) return rc;
   HIPCHK(hipStreamWaitEvent(S.side, S.ev_fork, 0));
   rc = run_on(S.side, "exact_shift_old", 2.0 * (2.0 * os + 2.0 * 16.0) / 16.0 * bytes_field(), [&] {
-    return launch_exact_shift(S.D, 2, qo, fo, S.XRo, S.d_f0, S.fc, S.d_der, S.d_y, dgx, dgz, sx0, sz0, sx1, sz1,
+    return launch_exact_shift(S.D, 2, qo, fo, S.XRo, S.d_f0, S.fc, S.d_der, S.PX, S.PZ, sx0, sz0, sx1, sz1,
                               S.side);
   });
   if (rc) return rc;
@@ -443,6 +452,8 @@ static void release() {
   if (S.ev_fork) (void)hipEventDestroy(S.ev_fork);
   if (S.ev_join) (void)hipEventDestroy(S.ev_join);
   for (double2 *p : S.XRo) if (p) (void)hipFree(p);
+  if (S.PX) (void)hipFree(S.PX);
+  if (S.PZ) (void)hipFree(S.PZ);
   S = State();
 }
 
@@ -527,6 +538,8 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
       held += rows * sizeof(double2);
     }
   }
+  HIPCHK(hipMalloc(&S.PX, (size_t)D.ny * D.NXm * sizeof(double2)));
+  HIPCHK(hipMalloc(&S.PZ, (size_t)D.ny * D.NZm * sizeof(double2)));
   // transform chunk: rows of the physical-space buffers Hv (3 fields) and Hp (6)
   const size_t per_row = 9 * (size_t)D.nzd * D.NXp * cs;
   size_t free_b = 0, total_b = 0;

@@ -45,4 +45,15 @@ HSTD Cx<float> ldf(const float2 *p) { float2 v = *p; return Cx<float>{v.x, v.y};
 
 #undef HSTD
 
+// Workgroups are dealt to the 8 XCDs round-robin by their linear id and every
+// XCD has its own L2.  This renumbers the linear id so that each XCD receives
+// a contiguous range: neighbouring blocks, which share cache lines, then run
+// on the same XCD.  Returns the new linear id of the calling block.
+__device__ __forceinline__ int xcd_linear() {
+  const int nx = gridDim.x, ny = gridDim.y, n = nx * ny * gridDim.z;
+  const int lid = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+  const int per = n / 8, rem = n % 8, x = lid % 8, s = lid / 8;
+  return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + s;
+}
+
 }  // namespace hst

@@ -106,10 +106,7 @@ struct XCfg {
 // then meet in the same L2 instead of fetching the line twice.  Returns
 // (tile, row, component) for a grid (ntile, nrow, nc).
 __device__ __forceinline__ void xcd_tile(int &tile, int &row, int &comp) {
-  const int nt = gridDim.x, nr = gridDim.y, n = nt * nr * gridDim.z;
-  const int lid = blockIdx.x + nt * (blockIdx.y + nr * blockIdx.z);
-  const int per = n / 8, rem = n % 8, x = lid % 8, s = lid / 8;
-  const int l = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + s;
+  const int nt = gridDim.x, nr = gridDim.y, l = xcd_linear();
   tile = l % nt;
   row = (l / nt) % nr;
   comp = l / (nt * nr);
@@ -363,10 +360,12 @@ __device__ __forceinline__ void xl_c2r_load(Cx<R> (&dst)[XL<M>::E], const Cx<R> 
   }
 }
 
-// two wavefronts per SIMD at least: the FP64 lines of M <= 512 fit in 256
-// VGPRs with a few bytes of spill instead of taking one wavefront per SIMD
+// occupancy floor: the FP64 lines of M <= 512 fit in 256 VGPRs (two
+// wavefronts per SIMD) with a few bytes of spill instead of taking one; the
+// FP32 lines are bound by the vector ALU and run 4 % faster at four
+// wavefronts with 36 bytes of spill than at three without
 #ifndef XLINE_F32_WAVES
-#define XLINE_F32_WAVES 3
+#define XLINE_F32_WAVES 4
 #endif
 template <typename R, int M>
 constexpr int xline_min_waves() { return sizeof(R) == 8 ? (M >= 1024 ? 1 : 2) : (M >= 1024 ? 2 : XLINE_F32_WAVES); }

@@ -110,9 +110,13 @@ int launch_d0solve(const Dims &D, int rhs_mode, int out_mode, const double2 *src
 // The exact shift of q[0..nq) (rhs1, rhs2, oldrhs1, oldrhs2): D0 solve at the
 // old shifts, node phase, ghosts at the new shifts, D0 re-weight.
 // q[k] is float2 where f32[k] is set.  XR[k]: workspace of ny - 2 rows.
+// PX, PZ: the node phases of launch_shift_phases for the substep's (dgx, dgz).
 int launch_exact_shift(const Dims &D, int nq, void *const *q, const int *f32, double2 *const *XR,
-                       const double *f0rows, const D0Const &fc, const double *der, const double *y, double dgx,
-                       double dgz, double sx0, double sz0, double sx1, double sz1, hipStream_t s);
+                       const double *f0rows, const D0Const &fc, const double *der, const double2 *PX,
+                       const double2 *PZ, double sx0, double sz0, double sx1, double sz1, hipStream_t s);
+// PX[i][ix] = exp(-i alfa0 ix dgx y_i) (ny x (nx+1)), PZ[i][iz+nz] = exp(-i beta0 iz dgz y_i) (ny x (2nz+1)).
+int launch_shift_phases(const Dims &D, const double *y, double dgx, double dgz, double2 *PX, double2 *PZ,
+                        hipStream_t s);
 // float2 <-> double2 copies of n complex values
 int launch_narrow(const double2 *src, float2 *dst, size_t n, hipStream_t s);
 int launch_widen(const float2 *src, double2 *dst, size_t n, hipStream_t s);

@@ -507,8 +507,11 @@ __device__ __forceinline__ void penta_line(const Dims &D, int col, double lam, c
 template <int K0, int K1, bool TAB>
 __global__ void __launch_bounds__(256)
     k_penta(Dims D, PentaArgs pa, double lam, const double *__restrict__ der, double sx, double sz) {
-  const int sys = blockIdx.y;
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  // the +iz and -iz columns of an ix lie in neighbouring blocks and read the
+  // same factor rows: keep neighbouring blocks on one XCD (one L2)
+  const int l = TAB ? xcd_linear() : blockIdx.x + gridDim.x * blockIdx.y;
+  const int sys = l / gridDim.x;
+  const int col = (l % gridDim.x) * blockDim.x + threadIdx.x;
   if (col >= D.ncol) return;
   if (sys == 0) penta_line<K0, TAB>(D, col, lam, der, pa.src[0], pa.dst[0], pa.ws[0], pa.tab[0], sx, sz);
   else penta_line<K1, TAB>(D, col, lam, der, pa.src[1], pa.dst[1], pa.ws[1], pa.tab[1], sx, sz);
@@ -718,11 +721,32 @@ struct QList {
   int f32[4];
 };
 
+// The node phases of the shift, separable in x and z:
+// PX[i][ix] = exp(-i alfa0 ix dgx y_i), PZ[i][iz + nz] = exp(-i beta0 iz dgz y_i).
+__global__ void __launch_bounds__(256) k_shift_phases(Dims D, const double *__restrict__ y, double dgx, double dgz,
+                                                      double2 *__restrict__ PX, double2 *__restrict__ PZ) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (k < D.NXm) {
+    st(PX + (size_t)i * D.NXm + k, cexpi(-(D.alfa0 * k) * dgx * y[i + 2]));
+  } else if (k < D.NXm + D.NZm) {
+    const int izm = k - D.NXm, iz = izm - D.nz;
+    st(PZ + (size_t)i * D.NZm + izm, cexpi(-(D.beta0 * iz) * dgz * y[i + 2]));
+  }
+}
+
+int launch_shift_phases(const Dims &D, const double *y, double dgx, double dgz, double2 *PX, double2 *PZ,
+                        hipStream_t s) {
+  dim3 grid((D.NXm + D.NZm + 255) / 256, D.ny);
+  hipLaunchKernelGGL(k_shift_phases, grid, dim3(256), 0, s, D, y, dgx, dgz, PX, PZ);
+  return hipGetLastError();
+}
+
 template <typename QT>
 __device__ __forceinline__ void shift_column(const Dims &D, const D0Const &fc, QT *__restrict__ q,
                                              double2 *__restrict__ XR, int col, const double *__restrict__ f0,
-                                             const double *__restrict__ der, const double *__restrict__ y, double dgx,
-                                             double dgz, double sx0s, double sz0s, double sx1s, double sz1s) {
+                                             const double *__restrict__ der, const double2 *__restrict__ PX,
+                                             const double2 *__restrict__ PZ, double sx0s, double sz0s, double sx1s,
+                                             double sz1s) {
   int ix, iz;
   col_ix_iz(D, col, ix, iz);
   const size_t nc = D.ncol;
@@ -738,10 +762,10 @@ __device__ __forceinline__ void shift_column(const Dims &D, const D0Const &fc, Q
   const double *rr = fc.rec;
   const C xr0 = sx0 - cph * (rr[3] * b1 + rr[4] * b2) - (rr[5] * b1 + rr[6] * b2);
   const C xr1 = sx1 - cph * (rr[9] * b1 + rr[10] * b2) - (rr[11] * b1 + rr[12] * b2);
-  const double th = -((D.alfa0 * ix) * dgx + (D.beta0 * iz) * dgz);
+  auto phase = [&](int i) { return ld(PX + (size_t)i * D.NXm + ix) * ld(PZ + (size_t)i * D.NZm + iz + D.nz); };
   const C ph1 = cexpi(-(D.alfa0 * ix * sx1s + D.beta0 * iz * sz1s));
-  const C xp0 = xr0 * cexpi(th * y[2]), xp1 = xr1 * cexpi(th * y[3]);
-  const C xpm2 = b1 * cexpi(th * y[m - 2 + 2]), xpm1 = b2 * cexpi(th * y[m - 1 + 2]);
+  const C xp0 = xr0 * phase(0), xp1 = xr1 * phase(1);
+  const C xpm2 = b1 * phase(m - 2), xpm1 = b2 * phase(m - 1);
   // window W[k] = x'(i + 1 + k)
   C W0 = xpm2, W1 = xpm1, W2 = xp0 * ph1, W3 = xp1 * ph1;
   C xk1 = b1, xk2 = b2;
@@ -749,7 +773,7 @@ __device__ __forceinline__ void shift_column(const Dims &D, const D0Const &fc, Q
         const C xk = xr - F0(i, 5) * yb1 - F0(i, 6) * yb2 - F0(i, 3) * xk1 - F0(i, 4) * xk2;
         xk2 = xk1;
         xk1 = xk;
-        const C xp = xk * cexpi(th * y[i + 2]);
+        const C xp = xk * phase(i);
         const int r = i + 2;
         C acc{0, 0};
         acc = acc + DER(der, r, 0, -2) * xp;
@@ -781,24 +805,24 @@ __device__ __forceinline__ void shift_column(const Dims &D, const D0Const &fc, Q
 
 __global__ void __launch_bounds__(256)
     k_exact_shift(Dims D, D0Const fc, QList ql, const double *__restrict__ f0, const double *__restrict__ der,
-                  const double *__restrict__ y, double dgx, double dgz, double sx0s, double sz0s, double sx1s,
-                  double sz1s) {
+                  const double2 *__restrict__ PX, const double2 *__restrict__ PZ, double sx0s, double sz0s,
+                  double sx1s, double sz1s) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= D.ncol) return;
   const int k = blockIdx.y;
   if (ql.f32[k])
-    shift_column<float2>(D, fc, (float2 *)ql.q[k], ql.xr[k], col, f0, der, y, dgx, dgz, sx0s, sz0s, sx1s, sz1s);
+    shift_column<float2>(D, fc, (float2 *)ql.q[k], ql.xr[k], col, f0, der, PX, PZ, sx0s, sz0s, sx1s, sz1s);
   else
-    shift_column<double2>(D, fc, (double2 *)ql.q[k], ql.xr[k], col, f0, der, y, dgx, dgz, sx0s, sz0s, sx1s, sz1s);
+    shift_column<double2>(D, fc, (double2 *)ql.q[k], ql.xr[k], col, f0, der, PX, PZ, sx0s, sz0s, sx1s, sz1s);
 }
 
 int launch_exact_shift(const Dims &D, int nq, void *const *q, const int *f32, double2 *const *XR,
-                       const double *f0rows, const D0Const &fc, const double *der, const double *y, double dgx,
-                       double dgz, double sx0, double sz0, double sx1, double sz1, hipStream_t s) {
+                       const double *f0rows, const D0Const &fc, const double *der, const double2 *PX,
+                       const double2 *PZ, double sx0, double sz0, double sx1, double sz1, hipStream_t s) {
   QList ql;
   for (int k = 0; k < nq; k++) { ql.q[k] = q[k]; ql.xr[k] = XR[k]; ql.f32[k] = f32[k]; }
   dim3 grid((D.ncol + 255) / 256, nq);
-  hipLaunchKernelGGL(k_exact_shift, grid, dim3(256), 0, s, D, fc, ql, f0rows, der, y, dgx, dgz, sx0, sz0, sx1, sz1);
+  hipLaunchKernelGGL(k_exact_shift, grid, dim3(256), 0, s, D, fc, ql, f0rows, der, PX, PZ, sx0, sz0, sx1, sz1);
   return hipGetLastError();
 }
 
