@@ -304,7 +304,9 @@ static int exact_shift(double dt_sub, double time) {
 }
 
 // The exact shift inside a time step: rhs on the main stream (the solves read
-// it next), oldrhs on the side stream, joined before the next buildrhs.
+// it next), oldrhs on the side stream, joined before the next buildrhs.  With
+// timing on both halves run on the main stream, so that every kernel's time
+// is its own.
 static int exact_shift_split(double dt_sub, double time) {
   const double dgx = S.deck.S * dt_sub, dgz = s2_integral(time, time + dt_sub);
   double sx0, sz0, sx1, sz1;
@@ -324,14 +326,16 @@ static int exact_shift_split(double dt_sub, double time) {
                               S.stream);
   });
   if (rc) return rc;
-  HIPCHK(hipStreamWaitEvent(S.side, S.ev_fork, 0));
-  rc = run_on(S.side, "exact_shift_old", 2.0 * (2.0 * os + 2.0 * 16.0) / 16.0 * bytes_field(), [&] {
-    return launch_exact_shift(S.D, 2, qo, fo, S.XRo, S.d_f0, S.fc, S.d_der, S.PX, S.PZ, sx0, sz0, sx1, sz1,
-                              S.side);
+  hipStream_t st = S.prof ? S.stream : S.side;
+  if (!S.prof) HIPCHK(hipStreamWaitEvent(S.side, S.ev_fork, 0));
+  rc = run_on(st, "exact_shift_old", 2.0 * (2.0 * os + 2.0 * 16.0) / 16.0 * bytes_field(), [&] {
+    return launch_exact_shift(S.D, 2, qo, fo, S.XRo, S.d_f0, S.fc, S.d_der, S.PX, S.PZ, sx0, sz0, sx1, sz1, st);
   });
   if (rc) return rc;
-  HIPCHK(hipEventRecord(S.ev_join, S.side));
-  S.side_pending = true;
+  if (!S.prof) {
+    HIPCHK(hipEventRecord(S.ev_join, S.side));
+    S.side_pending = true;
+  }
   return 0;
 }
 
