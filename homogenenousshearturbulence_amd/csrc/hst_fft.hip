@@ -72,11 +72,14 @@ void free_fft_tables(void *p) {
 
 // ------------------------------------------------------------------ launch shapes
 
-template <typename R, int N>
+// Lines per block of the z kernels, measured at nzd = 768: FP32 four (more
+// blocks per CU than eight, the XCD order keeps the tiles that share a line in
+// one L2); FP64 eight for the inverse transform, two for the forward one.
+template <typename R, int N, bool FWD>
 struct ZCfg {
   static constexpr int T = fft_threads(N);
   static constexpr int LDS_MAX = 100 * 1024;
-  static constexpr int G0 = (sizeof(R) == 8 ? 8 : 16);  // 128-byte transposed segments
+  static constexpr int G0 = sizeof(R) == 8 ? (FWD ? 2 : 8) : 4;
   // LDS line stride: lanes of the transposed accesses walk the lines (stride LS)
   // and the points (stride 1); LS = 16 dwords mod 64 spreads them over all banks
   static constexpr int DW = (int)sizeof(Cx<R>) / 4, PER = 64 / DW;
@@ -117,10 +120,10 @@ __device__ __forceinline__ void xcd_tile(int &tile, int &row, int &comp) {
 // z inverse transform of the three velocity components for the rows of a
 // chunk.  grid (x tiles of G modes, rows, component); output Hv[c][row][zp][ix].
 template <typename R, int NZD>
-__global__ void __launch_bounds__((ZCfg<R, NZD>::NT))
+__global__ void __launch_bounds__((ZCfg<R, NZD, false>::NT))
     k_zift(const double2 *__restrict__ V1, const double2 *__restrict__ V2, const double2 *__restrict__ V3,
            Cx<R> *__restrict__ Hv, Dims D, int row0, int nrows, const Cx<R> *__restrict__ tw) {
-  constexpr int G = ZCfg<R, NZD>::G, T = ZCfg<R, NZD>::T, LS = ZCfg<R, NZD>::LS;
+  constexpr int G = ZCfg<R, NZD, false>::G, T = ZCfg<R, NZD, false>::T, LS = ZCfg<R, NZD, false>::LS;
   __shared__ Cx<R> lds[G * LS];
   __shared__ Cx<R> twl[NZD];  // the twiddles: lanes of a stage read scattered entries
   for (int k = threadIdx.x; k < NZD; k += blockDim.x) twl[k] = tw[k];
@@ -153,10 +156,10 @@ __global__ void __launch_bounds__((ZCfg<R, NZD>::NT))
 // ------------------------------------------------------------------ k_zfft
 
 template <typename R, int NZD>
-__global__ void __launch_bounds__((ZCfg<R, NZD>::NT))
+__global__ void __launch_bounds__((ZCfg<R, NZD, true>::NT))
     k_zfft(const Cx<R> *__restrict__ Hp, Cx<R> *__restrict__ P0, size_t pstride, Dims D, int row0, int nrows,
            const Cx<R> *__restrict__ tw) {
-  constexpr int G = ZCfg<R, NZD>::G, T = ZCfg<R, NZD>::T, LS = ZCfg<R, NZD>::LS;
+  constexpr int G = ZCfg<R, NZD, true>::G, T = ZCfg<R, NZD, true>::T, LS = ZCfg<R, NZD, true>::LS;
   __shared__ Cx<R> lds[G * LS];
   __shared__ Cx<R> twl[NZD];
   for (int k = threadIdx.x; k < NZD; k += blockDim.x) twl[k] = tw[k];
@@ -453,7 +456,7 @@ __global__ void __launch_bounds__(64, (xline_min_waves<R, M>()))
 template <typename R, int NZD>
 static int run_zift(const Dims &D, const double2 *const V[3], void *Hv, int row0, int nrows, const Tables<R> &tb,
                     hipStream_t s) {
-  typedef ZCfg<R, NZD> C;
+  typedef ZCfg<R, NZD, false> C;
   dim3 grid((D.NXm + C::G - 1) / C::G, nrows, 3);
   hipLaunchKernelGGL((k_zift<R, NZD>), grid, dim3(C::NT), 0, s, V[0], V[1], V[2], (Cx<R> *)Hv, D, row0, nrows,
                      tb.twz);
@@ -463,7 +466,7 @@ static int run_zift(const Dims &D, const double2 *const V[3], void *Hv, int row0
 template <typename R, int NZD>
 static int run_zfft(const Dims &D, int np, const void *Hp, void *P0, size_t pstride, int row0, int nrows,
                     const Tables<R> &tb, hipStream_t s) {
-  typedef ZCfg<R, NZD> C;
+  typedef ZCfg<R, NZD, true> C;
   dim3 grid((D.NXm + C::G - 1) / C::G, nrows, np);
   hipLaunchKernelGGL((k_zfft<R, NZD>), grid, dim3(C::NT), 0, s, (const Cx<R> *)Hp, (Cx<R> *)P0, pstride, D, row0,
                      nrows, tb.twz);
