@@ -76,6 +76,9 @@ struct State {
   bool side_pending = false;
   double2 *XRo[2] = {nullptr, nullptr};
   double2 *PX = nullptr, *PZ = nullptr;  // node phases of the exact shift
+  double *exz_acc = nullptr;             // column energies summed over the outstats calls
+  int exz_n = 0;
+  double mean_acc = 0.0;                 // summed squared box means of the (0,0) column
   int64_t bytes_held = 0;
   bool prof = false;
   std::vector<Pending> pending;
@@ -457,6 +460,7 @@ static void release() {
   if (S.ev_join) (void)hipEventDestroy(S.ev_join);
   for (double2 *p : S.XRo) if (p) (void)hipFree(p);
   if (S.PX) (void)hipFree(S.PX);
+  if (S.exz_acc) (void)hipFree(S.exz_acc);
   if (S.PZ) (void)hipFree(S.PZ);
   S = State();
 }
@@ -542,6 +546,8 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
       held += rows * sizeof(double2);
     }
   }
+  HIPCHK(hipMalloc(&S.exz_acc, (size_t)D.ncol * sizeof(double)));
+  HIPCHK(hipMemset(S.exz_acc, 0, (size_t)D.ncol * sizeof(double)));
   HIPCHK(hipMalloc(&S.PX, (size_t)D.ny * D.NXm * sizeof(double2)));
   HIPCHK(hipMalloc(&S.PZ, (size_t)D.ny * D.NZm * sizeof(double2)));
   // transform chunk: rows of the physical-space buffers Hv (3 fields) and Hp (6)
@@ -741,7 +747,7 @@ int hst_gpu_outstats(double time, double out[10]) {
   if (int rc = check_live()) return rc;
   int nb = 0;
   int rc = run("stats", 3.0 * bytes_field(), [&] {
-    return launch_stats(S.D, S.F[1], S.F[2], S.F[3], S.d_dyl, S.d_part, &nb, S.stream);
+    return launch_stats(S.D, S.F[1], S.F[2], S.F[3], S.d_dyl, S.d_part, &nb, S.exz_acc, S.stream);
   });
   if (rc) return rc;
   std::vector<double> h((size_t)nb * 9);
@@ -771,6 +777,8 @@ int hst_gpu_outstats(double time, double out[10]) {
   uu = uu - mfx * mfx / ly_w;
   ww = ww - mfz * mfz / ly_w;
   uw = uw - mfx * mfz / ly_w;
+  S.mean_acc += (mfx * mfx + mfz * mfz) / ly_w;
+  S.exz_n += 1;
   const double q2 = uu + vv + ww, inv_ly = 1.0 / S.deck.ly;
   out[0] = 0.5 * q2 * inv_ly;
   out[1] = S.g.ni * eps * inv_ly;
@@ -783,6 +791,55 @@ int hst_gpu_outstats(double time, double out[10]) {
   out[8] = mfx;
   out[9] = mfz;
   return 0;
+}
+
+int hst_gpu_spectrum(double *E, int32_t *nacc) {
+  if (int rc = check_live()) return rc;
+  if (!E || !nacc) return fail("spectrum: NULL argument");
+  const size_t n = S.D.ncol;
+  HIPCHK(hipMemcpyAsync(E, S.exz_acc, n * sizeof(double), hipMemcpyDeviceToHost, S.stream));
+  HIPCHK(hipMemsetAsync(S.exz_acc, 0, n * sizeof(double), S.stream));
+  HIPCHK(hipStreamSynchronize(S.stream));
+  *nacc = S.exz_n;
+  // average; the (0,0) column about its box mean; E = half the column energy
+  // (spectrum_write, src/hst_stats.f90:110-134)
+  if (S.exz_n > 0) {
+    for (size_t c = 0; c < n; c++) E[c] /= S.exz_n;
+    E[S.D.nz] -= S.mean_acc / S.exz_n;
+  } else {
+    for (size_t c = 0; c < n; c++) E[c] = 0.0;
+  }
+  for (size_t c = 0; c < n; c++) E[c] *= 0.5;
+  S.exz_n = 0;
+  S.mean_acc = 0.0;
+  return 0;
+}
+
+int hst_gpu_pressure(double time, int dst) {
+  if (int rc = check_live()) return rc;
+  if (int rc = check_field64(dst)) return rc;
+  if (dst == HST_V1 || dst == HST_V2 || dst == HST_V3 || dst == HST_RHS1)
+    return fail("pressure: dst must not be u, v, w or rhs1 (the right-hand side's buffer)");
+  if (S.deck.linear) return fail("pressure: the linear equations have no products");
+  int rc = nonlinear_pass(0, false, nullptr);
+  if (rc) return rc;
+  const double ps = S.mixed ? 8.0 : 16.0;
+  rc = run("pressure_rhs", (double)S.D.ny * S.D.ncol * (6.0 * ps + 16.0 + 16.0), [&] {
+    return launch_pressure_rhs(S.D, S.mixed, S.P, S.F[HST_V2], S.F[HST_RHS1], S.d_der, s2_of(time), S.stream);
+  });
+  if (rc) return rc;
+  double sx, sz;
+  shear_shifts(time, sx, sz);
+  const int kinds[1] = {HST_KIND_POISSON};
+  const double2 *src[1] = {S.F[HST_RHS1]};
+  double2 *dsts[1] = {S.F[dst]};
+  rc = run("penta", 4.0 * bytes_field(), [&] {
+    return launch_penta(S.D, 1, kinds, 0.0, src, dsts, S.pws, nullptr, S.d_der, sx, sz, S.stream);
+  });
+  if (rc) return rc;
+  return run("pressure_mean", 0.0, [&] {
+    return launch_pressure_mean(S.D, S.F[HST_RHS1], S.F[dst], S.d_dyl, S.deck.ly, S.stream);
+  });
 }
 
 int hst_gpu_profile(int enable) {

@@ -125,8 +125,14 @@ int launch_widen(const float2 *src, double2 *dst, size_t n, hipStream_t s);
 int launch_ghosts(const Dims &D, int nf, double2 *const *f, double sx, double sz, hipStream_t s);
 
 // First reduction of the statistics: per block 9 sums (uu vv ww uv vw uw eps mfx mfz).
+// exz_acc (may be null): the column energies sum_y w |u|^2 are added to it.
 int launch_stats(const Dims &D, const double2 *V1, const double2 *V2, const double2 *V3, const double *dyl,
-                 double *part, int *n_part, hipStream_t s);
+                 double *part, int *n_part, double *exz_acc, hipStream_t s);
+// The pressure equation's right-hand side from the six product spectra and v
+// into R (interior rows), and the (0,0) column of p from it.
+int launch_pressure_rhs(const Dims &D, int mixed, const void *P0, const double2 *V2, double2 *R, const double *der,
+                        double s2now, hipStream_t s);
+int launch_pressure_mean(const Dims &D, const double2 *R, double2 *p, const double *dyl, double ly, hipStream_t s);
 
 // Layout conversion between the reference layout (host) and the device layout.
 int launch_to_device_layout(const Dims &D, const double2 *ref, double2 *dev, hipStream_t s);

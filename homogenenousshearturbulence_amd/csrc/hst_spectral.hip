@@ -865,7 +865,7 @@ int launch_ghosts(const Dims &D, int nf, double2 *const *f, double sx, double sz
 // Stokes layer and damping: per block the sums uu vv ww uv vw uw eps mfx mfz.
 __global__ void __launch_bounds__(256)
     k_stats(Dims D, const double2 *__restrict__ V1, const double2 *__restrict__ V2, const double2 *__restrict__ V3,
-            const double *__restrict__ dyl, double *__restrict__ part) {
+            const double *__restrict__ dyl, double *__restrict__ part, double *__restrict__ exz) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (col < D.ncol) {
@@ -873,10 +873,14 @@ __global__ void __launch_bounds__(256)
     col_ix_iz(D, col, ix, iz);
     const double al = D.alfa0 * ix, be = D.beta0 * iz, kk = al * al + be * be;
     const size_t nc = D.ncol;
+    // the column energy sum_y w |u|^2 of the spectrum (src/hst_stats.f90:225-236), every column
+    double qc = 0.0;
     for (int iy = 0; iy < D.ny; iy++) {
       const size_t o = (size_t)(iy + 2) * nc + col;
       const C cu = ld(V1 + o), cv = ld(V2 + o), cw = ld(V3 + o);
       const double dy = dyl[iy];
+      qc = qc + (ix == 0 ? dy : 2.0 * dy) * (cu.x * cu.x + cu.y * cu.y + cv.x * cv.x + cv.y * cv.y + cw.x * cw.x +
+                                             cw.y * cw.y);
       if (ix == 0 && iz == 0) {
         s[7] += cu.x * dy;
         s[8] += cw.x * dy;
@@ -895,6 +899,7 @@ __global__ void __launch_bounds__(256)
       s[5] += w * (cu.x * cw.x + cu.y * cw.y);
       s[6] += w * kk * (uu + vv + wv);
     }
+    if (exz) exz[col] += qc;
   }
   __shared__ double red[4][9];
 #pragma unroll
@@ -911,10 +916,89 @@ __global__ void __launch_bounds__(256)
 }
 
 int launch_stats(const Dims &D, const double2 *V1, const double2 *V2, const double2 *V3, const double *dyl,
-                 double *part, int *n_part, hipStream_t s) {
+                 double *part, int *n_part, double *exz_acc, hipStream_t s) {
   dim3 grid((D.ncol + 255) / 256);
   if (n_part) *n_part = grid.x;
-  hipLaunchKernelGGL(k_stats, grid, dim3(256), 0, s, D, V1, V2, V3, dyl, part);
+  hipLaunchKernelGGL(k_stats, grid, dim3(256), 0, s, D, V1, V2, V3, dyl, part, exz_acc);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ pressure
+
+// The right-hand side of the pressure equation of one column (compute_pressure,
+// src/hst_pressure.f90:35-88):
+//   D0 [alfa^2 uu + beta^2 ww + 2 alfa beta uw] - D2 vv - 2 i alfa D1 uv
+//   - 2 i beta D1 vw - 2 i (S alfa + S2 beta) D0 v,
+// the (0,0) column -<vv>, the raw product.  Stencils of the products in PR.
+template <typename PR>
+__global__ void __launch_bounds__(256)
+    k_pressure_rhs(Dims D, const typename PTraits<PR>::T2 *__restrict__ P0, size_t pstride,
+                   const double2 *__restrict__ V2, double2 *__restrict__ R, const double *__restrict__ der,
+                   double s2now) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= D.ncol) return;
+  int ix, iz;
+  col_ix_iz(D, col, ix, iz);
+  const double al = D.alfa0 * ix, be = D.beta0 * iz;
+  const bool m00 = (ix == 0 && iz == 0);
+  const size_t nc = D.ncol;
+  const int iy = blockIdx.y;
+  auto sten = [&](int q, int n) {
+    Cx<PR> a{0, 0};
+#pragma unroll
+    for (int j = -2; j <= 2; j++) a = a + (PR)DER(der, iy, n, j) * ldp<PR>(P0 + q * pstride + (size_t)(iy + j + 2) * nc + col);
+    return C{(double)a.x, (double)a.y};
+  };
+  C d0v{0, 0};
+#pragma unroll
+  for (int j = -2; j <= 2; j++) d0v = d0v + DER(der, iy, 0, j) * ld(V2 + (size_t)(iy + j + 2) * nc + col);
+  C r = (al * al) * sten(0, 0) - imul(2.0 * (D.S * al + s2now * be), d0v);
+  r = r - sten(1, 2);
+  if (m00) {
+    const Cx<PR> vv = ldp<PR>(P0 + 1 * pstride + (size_t)(iy + 2) * nc + col);
+    r = C{-(double)vv.x, 0.0};
+  }
+  r = r + (be * be) * sten(2, 0);
+  r = r - imul(2.0 * al, sten(3, 1));
+  r = r - imul(2.0 * be, sten(4, 1));
+  r = r + (2.0 * al * be) * sten(5, 0);
+  st(R + (size_t)(iy + 2) * nc + col, r);
+}
+
+int launch_pressure_rhs(const Dims &D, int mixed, const void *P0, const double2 *V2, double2 *R, const double *der,
+                        double s2now, hipStream_t s) {
+  dim3 grid((D.ncol + 255) / 256, D.ny);
+  const size_t pstride = (size_t)D.NYg * D.ncol;
+  if (mixed)
+    hipLaunchKernelGGL(k_pressure_rhs<float>, grid, dim3(256), 0, s, D, (const float2 *)P0, pstride, V2, R, der, s2now);
+  else
+    hipLaunchKernelGGL(k_pressure_rhs<double>, grid, dim3(256), 0, s, D, (const double2 *)P0, pstride, V2, R, der,
+                       s2now);
+  return hipGetLastError();
+}
+
+// The singular (0,0) mode (src/hst_pressure.f90:94-111): p00 = the right-hand
+// side itself, minus its y mean sum(Re p00 dyl)/ly; one block, a fixed order.
+__global__ void __launch_bounds__(256)
+    k_pressure_mean(Dims D, const double2 *__restrict__ R, double2 *__restrict__ p, const double *__restrict__ dyl,
+                    double ly) {
+  __shared__ double red[256];
+  const size_t c00 = (size_t)D.nz, nc = D.ncol;
+  double a = 0.0;
+  for (int iy = threadIdx.x; iy < D.ny; iy += 256) a += R[(size_t)(iy + 2) * nc + c00].x * dyl[iy];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double pmean = red[0] / ly;
+  for (int iy = threadIdx.x; iy < D.ny; iy += 256)
+    p[(size_t)(iy + 2) * nc + c00] = make_double2(R[(size_t)(iy + 2) * nc + c00].x - pmean, 0.0);
+}
+
+int launch_pressure_mean(const Dims &D, const double2 *R, double2 *p, const double *dyl, double ly, hipStream_t s) {
+  hipLaunchKernelGGL(k_pressure_mean, dim3(1), dim3(256), 0, s, D, R, p, dyl, ly);
   return hipGetLastError();
 }
 

@@ -1,6 +1,7 @@
 ! Direct numerical simulation of homogeneous shear turbulence: the driver of
 ! the reference (src/hst.f90) on one MI355X.  Usage:  hst [hst.in]
-! Set-up, then per step: timestep(), statistics, snapshots, restart file, new
+! Set-up, then per step: timestep(), statistics, snapshots (velocity,
+! pressure and the averaged column-energy spectrum), restart file, new
 ! time step from the CFL number.  The run ends at t_max, after nstep steps or
 ! after wall_max seconds, whichever comes first.
 program hst
@@ -17,6 +18,7 @@ program hst
   use hst_initial, only: generate_initial_field
   use hst_io
   use hst_stats
+  use hst_pressure
   use hst_timer
 
   implicit none
@@ -70,6 +72,10 @@ program hst
       write (fname, '(A,I0,A)') 'fields/field', ifield, '.fld'
       if (has_terminal) print '(A,F12.5)', '   writing '//trim(fname)//' at time', time
       call restart_write(trim(fname))
+      write (fname, '(A,I0,A)') 'p_fields/pField', ifield, '.fld'
+      call write_pressure(trim(fname))
+      write (fname, '(A,I0,A)') 'statistics/spectra/exz', ifield, '.dat'
+      call spectrum_write(trim(fname))
     end if
     if (crossed(dt_save)) then
       if (has_terminal) print '(A,F12.5)', '   writing Dati.cart.out at time', time

@@ -135,6 +135,16 @@ module hst_gpu_api
       real(c_double), value :: time
       real(c_double), intent(out) :: out(10)
     end function
+    integer(c_int) function hst_gpu_pressure(time, dst) bind(c, name='hst_gpu_pressure')
+      import :: c_int, c_double
+      real(c_double), value :: time
+      integer(c_int), value :: dst
+    end function
+    integer(c_int) function hst_gpu_spectrum(E, nacc) bind(c, name='hst_gpu_spectrum')
+      import :: c_int, c_int32_t, c_double
+      real(c_double), intent(out) :: E(*)
+      integer(c_int32_t), intent(out) :: nacc
+    end function
     integer(c_int) function hst_gpu_set_chunk_rows(rows) bind(c, name='hst_gpu_set_chunk_rows')
       import :: c_int
       integer(c_int), value :: rows

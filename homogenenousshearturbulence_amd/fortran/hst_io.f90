@@ -15,7 +15,7 @@ module hst_io
 
   implicit none
   private
-  public :: restart_read, restart_write, make_output_dirs, download_velocity
+  public :: restart_read, restart_write, make_output_dirs, download_velocity, field_write
 
   character, parameter :: LF = achar(10), TAB = achar(9)
   integer, parameter :: CPL_ORDER(3) = [1, 3, 2]     ! our component behind CPL's (u, v, w)
@@ -133,6 +133,31 @@ contains
     close (unit)
     deallocate (buf)
   end subroutine restart_write
+
+  ! One device field as a headerless CPL file of one component (field_write,
+  ! src/hst_io.f90:273-289): rows 0..ny-1, then ny, ny+1 as copies of 0, 1
+  ! and -2, -1 as copies of ny-2, ny-1, in the file order (-2..ny+1, -nz..nz,
+  ! 0..nx) with y fastest.
+  subroutine field_write(filename, fid)
+    character(len=*), intent(in) :: filename
+    integer(C_INT), intent(in) :: fid
+    complex(C_DOUBLE_COMPLEX), allocatable :: f(:, :, :), buf(:, :, :)
+    integer(C_INT) :: ix, iz
+    integer :: unit
+    allocate (f(-2:ny + 1, -nz:nz, 0:nx), buf(-2:ny + 1, -nz:nz, 0:nx))
+    call gpu_check(hst_gpu_download_field(fid, f), 'download field')
+    do ix = 0, nx
+      do iz = -nz, nz
+        buf(0:ny - 1, iz, ix) = f(0:ny - 1, iz, ix)
+        buf(ny:ny + 1, iz, ix) = f(0:1, iz, ix)
+        buf(-2:-1, iz, ix) = f(ny - 2:ny - 1, iz, ix)
+      end do
+    end do
+    open (newunit=unit, file=trim(filename), access='stream', form='unformatted', status='replace', action='write')
+    write (unit) buf
+    close (unit)
+    deallocate (f, buf)
+  end subroutine field_write
 
   real(C_DOUBLE) function header_value(head, key)
     character(len=*), intent(in) :: head, key

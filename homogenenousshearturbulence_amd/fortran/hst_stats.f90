@@ -1,6 +1,8 @@
 ! Runtime statistics in the layout of the CPL code (the reference's module
 ! hst_stats): statistics/Runtimedata and statistics/variances_runtime.dat,
-! opened for append.  The reductions run on the GPU (hst_gpu_outstats).
+! opened for append, and the column-energy spectra statistics/spectra/exz<n>.dat.
+! The reductions and the column energies' running sum stay on the GPU
+! (hst_gpu_outstats, hst_gpu_spectrum).
 module hst_stats
 
   use, intrinsic :: iso_c_binding
@@ -10,16 +12,44 @@ module hst_stats
 
   implicit none
   private
-  public :: outstats, open_runtimedata, close_runtimedata
+  public :: outstats, open_runtimedata, close_runtimedata, spectrum_write
 
   integer, parameter :: unit_rt = 121, unit_var = 122
+  real(C_DOUBLE), save :: t_acc0 = 0.0d0     ! start of the running spectrum average
 
 contains
 
   subroutine open_runtimedata()
     if (has_terminal) open (unit=unit_rt, file='statistics/Runtimedata', action='write', position='append')
     if (has_terminal) open (unit=unit_var, file='statistics/variances_runtime.dat', action='write', position='append')
+    t_acc0 = time
   end subroutine open_runtimedata
+
+  ! The column energies averaged since the previous call (or the start), with
+  ! the (0,0) column about its box mean, then a new average (spectrum_write,
+  ! src/hst_stats.f90:110-134; the text layout is the reference's).
+  subroutine spectrum_write(filename)
+    character(len=*), intent(in) :: filename
+    real(C_DOUBLE), allocatable :: E(:, :)
+    integer(C_INT32_T) :: nacc
+    integer(C_INT) :: ix, iz
+    integer :: unit
+    allocate (E(-nz:nz, 0:nx))
+    call gpu_check(hst_gpu_spectrum(E, nacc), 'spectrum')
+    if (has_terminal) then
+      open (newunit=unit, file=filename, action='write')
+      write (unit, '(A,F12.5,A,F12.5,A,I0,A)') '# column energies ly/2 sum_y |u|^2 averaged over t = ', t_acc0, &
+        ' .. ', time, ' (', nacc, ' samples);  ix  iz  kx  kz  E'
+      do ix = 0, nx
+        do iz = -nz, nz
+          write (unit, '(I5,1X,I5,1X,ES15.7,1X,ES15.7,1X,ES15.7)') ix, iz, alfa0*ix, beta0*iz, E(iz, ix)
+        end do
+      end do
+      close (unit)
+    end if
+    deallocate (E)
+    t_acc0 = time
+  end subroutine spectrum_write
 
   subroutine close_runtimedata()
     if (has_terminal) close (unit_rt)

@@ -33,7 +33,7 @@ EXPORTS = (
     "hst_gpu_nonlinear", "hst_gpu_compute_cfl", "hst_gpu_product_to_field", "hst_gpu_roundtrip",
     "hst_gpu_buildrhs_prepare", "hst_gpu_buildrhs", "hst_gpu_shear_shift", "hst_gpu_line_solve", "hst_gpu_linsolve",
     "hst_gpu_timestep", "hst_gpu_outstats", "hst_gpu_set_chunk_rows", "hst_gpu_memory", "hst_gpu_profile",
-    "hst_gpu_profile_count", "hst_gpu_profile_entry",
+    "hst_gpu_profile_count", "hst_gpu_profile_entry", "hst_gpu_pressure", "hst_gpu_spectrum",
 )
 
 
@@ -82,6 +82,8 @@ def lib() -> ctypes.CDLL:
         "hst_gpu_profile": [i],
         "hst_gpu_profile_count": [],
         "hst_gpu_profile_entry": [i, ctypes.c_char_p, i, ctypes.POINTER(ctypes.c_int64), dp, dp],
+        "hst_gpu_pressure": [d, i],
+        "hst_gpu_spectrum": [vp, ctypes.POINTER(ctypes.c_int32)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -274,6 +276,22 @@ class Solver:
         out = np.zeros(10)
         _check(lib().hst_gpu_outstats(self.time, _dp(out)), "hst_gpu_outstats")
         return out
+
+    def pressure(self, dst: int = 5) -> np.ndarray:
+        """The fluctuation pressure of the current velocity at self.time
+        (compute_pressure, src/hst_pressure.f90:35-112), solved into field dst
+        (default HST_RHS2) and returned."""
+        _check(lib().hst_gpu_pressure(self.time, dst), "hst_gpu_pressure")
+        return self.download(dst)
+
+    def spectrum(self):
+        """(E[ix, iz + nz], n): half the column energies averaged over the
+        outstats calls since the previous call (spectrum_write,
+        src/hst_stats.f90:110-134); restarts the average."""
+        E = np.zeros((self.deck.nx + 1, 2 * self.deck.nz + 1))
+        n = ctypes.c_int32(0)
+        _check(lib().hst_gpu_spectrum(E.ctypes.data, ctypes.byref(n)), "hst_gpu_spectrum")
+        return E, n.value
 
     def sync(self) -> None:
         _check(lib().hst_gpu_sync(), "hst_gpu_sync")

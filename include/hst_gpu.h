@@ -152,6 +152,20 @@ int hst_gpu_timestep(double deltat, double *time, double *cfl);
  * variances_runtime.dat), mfx, mfz. */
 int hst_gpu_outstats(double time, double out[10]);
 
+/* The fluctuation pressure of the current velocity (ghost rows filled) at
+ * `time` into field dst (compute_pressure, src/hst_pressure.f90:35-112):
+ * the products once more, the right-hand side in HST_RHS1, a POISSON line
+ * solve, the (0,0) mode -<vv> about its y mean.  dst: HST_RHS2, HST_SCRATCH or
+ * (FP64 form) HST_OLD1/2; outside a time step only. */
+int hst_gpu_pressure(double time, int dst);
+
+/* The column energies E(ix, iz) = ly/2 sum_y |u|^2 (the (0,0) column about
+ * its box mean) averaged over the hst_gpu_outstats calls since the previous
+ * call (spectrum_write, src/hst_stats.f90:110-134): E[ncol] in the column
+ * order (iz + nz) + (2nz+1) ix, *nacc the number of samples; restarts the
+ * average. */
+int hst_gpu_spectrum(double *E, int32_t *nacc);
+
 /* Rows per transform chunk (0: as many as memory allows; the physical-space
  * buffers hold one chunk of y rows).  Takes effect at the next hst_gpu_init. */
 int hst_gpu_set_chunk_rows(int rows);

@@ -21,6 +21,10 @@
  *   der               polynomial exactness of each stencil
  *   initial field     integer hash restated exactly; divergence-free; the fixtures
  *   outstats          direct recomputation, box-mean removal (test_stats)
+ *   pressure          Taylor-Green closed form (tests/test_pressure.f90: the two
+ *                     nonzero modes to 1e-4, the rest to 1e-10); the shear term
+ *                     against the inverse of the continuous operator on one mode
+ *   column energy     sum over the columns = the energy of outstats; one mode
  */
 #include <complex.h>
 #include <math.h>
@@ -984,6 +988,84 @@ void hsto_outstats_at(hsto *o, double time, double out[10]) {
   out[7] = uw * inv_ly;
   out[8] = mfx;
   out[9] = mfz;
+}
+
+/* compute_pressure (src/hst_pressure.f90:35-112) into field dst at `time`
+ * (the velocity's ghost rows filled): the products once more, the pressure
+ * right-hand side into HST_RHS1, a POISSON line solve, and the singular
+ * (0,0) mode set to -<vv> about its y mean. */
+void hsto_pressure(hsto *o, double time, int dst) {
+  hsto_nonlinear(o, 0, 0);
+  const double s2now = s2_of(o, time), S = o->d.S;
+  const cplx *V2 = o->f[2];
+  cplx *R = o->f[HST_RHS1];
+  const size_t nf = (size_t)o->NXm * o->NZm * o->NYg;
+#pragma omp parallel for collapse(2)
+  for (int ix = 0; ix <= o->nx; ix++)
+    for (int iz = -o->nz; iz <= o->nz; iz++) {
+      const cplx ialfa = I * (ix * o->d.alfa0), ibeta = I * (iz * o->d.beta0);
+      for (int iy = 0; iy < o->ny; iy++) {
+        size_t c = FIDX(o, ix, iz, iy);
+        cplx r = 0.0;
+        for (int m = 1; m <= 6; m++) {
+          const cplx *P = o->prod + (size_t)(m - 1) * nf;
+          cplx d0 = 0.0, d1 = 0.0, d2 = 0.0, term;
+          for (int j = -2; j <= 2; j++) {
+            d0 = d0 + DER(o, iy, 0, j) * P[c + j];
+            d1 = d1 + DER(o, iy, 1, j) * P[c + j];
+            d2 = d2 + DER(o, iy, 2, j) * P[c + j];
+          }
+          switch (m) {
+            case 1: term = -(ialfa * ialfa * d0); break;
+            case 2: term = -d2; break;
+            case 3: term = -(ibeta * ibeta * d0); break;
+            case 4: term = -2.0 * ialfa * d1; break;
+            case 5: term = -2.0 * ibeta * d1; break;
+            default: term = -2.0 * ialfa * ibeta * d0; break;
+          }
+          if (m == 1) {
+            cplx v0 = 0.0;
+            for (int j = -2; j <= 2; j++) v0 = v0 + DER(o, iy, 0, j) * V2[c + j];
+            r = term - 2.0 * (S * ialfa + s2now * ibeta) * v0;
+          } else {
+            r = r + term;
+          }
+          if (ix == 0 && iz == 0 && m == 2) r = -creal(P[c]);
+        }
+        R[c] = r;
+      }
+    }
+  double sx, sz;
+  hsto_shear_shifts(o, time, &sx, &sz);
+  hsto_line_solve(o, HST_KIND_POISSON, 0.0, HST_RHS1, dst, sx, sz);
+  cplx *p = o->f[dst];
+  double pmean = 0.0;
+  for (int iy = 0; iy < o->ny; iy++) {
+    size_t c = FIDX(o, 0, 0, iy);
+    p[c] = R[c];
+    pmean += creal(p[c]) * o->dyl[iy];
+  }
+  pmean = pmean / o->d.ly;
+  for (int iy = 0; iy < o->ny; iy++) {
+    size_t c = FIDX(o, 0, 0, iy);
+    p[c] = creal(p[c]) - pmean;
+  }
+}
+
+/* The column energies of outstats (src/hst_stats.f90:224-238):
+ * E[ix][iz + nz] = sum_y w |u|^2 + |v|^2 + |w|^2, w = dyl (ix = 0) or 2 dyl. */
+void hsto_column_energy(const hsto *o, double *E) {
+  const cplx *V1 = o->f[1], *V2 = o->f[2], *V3 = o->f[3];
+  for (int ix = 0; ix <= o->nx; ix++)
+    for (int iz = -o->nz; iz <= o->nz; iz++) {
+      double qc = 0.0;
+      for (int iy = 0; iy < o->ny; iy++) {
+        size_t c = FIDX(o, ix, iz, iy);
+        double w = ix == 0 ? o->dyl[iy] : 2.0 * o->dyl[iy];
+        qc += w * creal(V1[c] * conj(V1[c]) + V2[c] * conj(V2[c]) + V3[c] * conj(V3[c]));
+      }
+      E[(size_t)ix * o->NZm + (iz + o->nz)] = qc;
+    }
 }
 
 void hsto_set_threads(int n) {

@@ -62,6 +62,8 @@ def lib():
         L.hsto_outstats_at.argtypes = [vp, d, dp]
         L.hsto_set_threads.argtypes = [i]
         L.hsto_max_threads.restype = i
+        L.hsto_pressure.argtypes = [vp, d, i]
+        L.hsto_column_energy.argtypes = [vp, vp]
         # learned closure
         L.hsto_filter_weights.restype = i
         L.hsto_filter_weights.argtypes = [i, i, vp, vp]
@@ -158,6 +160,17 @@ class Oracle:
         lib().hsto_timestep(self.h, deltat, ctypes.byref(t), ctypes.byref(c))
         self.time = t.value
         self.cfl = c.value
+
+    def pressure(self, dst: int = 5) -> np.ndarray:
+        """compute_pressure at the current time into field dst (default HST_RHS2)."""
+        lib().hsto_pressure(self.h, self.time, dst)
+        return self.field(dst).copy()
+
+    def column_energy(self) -> np.ndarray:
+        """E[ix, iz + nz] of the current velocity (one outstats sample)."""
+        E = np.zeros((self.deck.nx + 1, 2 * self.deck.nz + 1))
+        lib().hsto_column_energy(self.h, E.ctypes.data)
+        return E
 
     def outstats(self) -> np.ndarray:
         out = np.zeros(10)

@@ -139,3 +139,67 @@ def forcing(make_solver, deck):
     ref = float(np.abs(expected - eta0).max())
     errv = float(np.abs(Vn[1, mx, deck.nz, 2:deck.ny + 2]).max())
     return err / ref, errv
+
+
+def _tg_plane_field(deck, y, mx=3, ky=2.0 * PI):
+    """u = cos(kx) sin(ky), v = -sin(kx) cos(ky) as stored modes (ix = mx)."""
+    V = np.zeros((3, deck.nx + 1, 2 * deck.nz + 1, deck.ny + 4), dtype=np.complex128)
+    V[0, mx, deck.nz] = 0.5 * np.sin(ky * y)
+    V[1, mx, deck.nz] = 0.5j * np.cos(ky * y)
+    return V
+
+
+def pressure_taylor_green(make_solver, deck):
+    """test_pressure (tests/test_pressure.f90:32-100): the x-y Taylor-Green
+    vortex (S = 0) has p = -(cos 2kx + cos 2ky)/4: p(2 mx, 0) = -1/8 and
+    p(0, 0) = -cos(2 ky y)/4 (zero-mean gauge).  Returns (max error of those
+    two modes, max magnitude of every other mode), interior rows."""
+    deck = dataclasses.replace(deck, S=0.0, linear=False, time=0.0)
+    mx, my = 3, 2
+    assert abs(mx * deck.alfa0 - 2 * PI * my / deck.ly) < 1e-12
+    s = make_solver(deck)
+    s.time = 0.0
+    ky = mx * deck.alfa0
+    s.set_V(_tg_plane_field(deck, s.y, mx, ky))
+    s.fill_all_ghosts()
+    p = s.pressure()[..., 2:deck.ny + 2]
+    yy = s.y[2:deck.ny + 2]
+    nz = deck.nz
+    err = max(float(np.abs(p[2 * mx, nz] + 0.125).max()),
+              float(np.abs(p[0, nz] + 0.25 * np.cos(2.0 * ky * yy)).max()))
+    rest = p.copy()
+    rest[2 * mx, nz] = 0.0
+    rest[0, nz] = 0.0
+    return err, float(np.abs(rest).max())
+
+
+def pressure_shear(make_solver, deck, s2=0.5):
+    """The mean-shear source of compute_pressure (src/hst_pressure.f90:1-9,
+    the -2 (S i alfa + S2 i beta) D0 v term) on two single modes of v,
+    v = A cos(ky y) at (ix, iz) = (mx, 0) and (mx, 1), u = w = 0: no product
+    reaches these two modes, so each solves (d_yy - k^2) p = -2 i (S alfa +
+    S2 beta) v exactly: p = 2 i (S alfa + S2 beta) A cos(ky y) / (k^2 + ky^2).
+    Returns the max error over both modes relative to the larger amplitude."""
+    deck = dataclasses.replace(deck, S=1.0, linear=False, time=0.0, s2_mean=s2, s2_amplitude=0.0,
+                               s2_period=0.0, s2_start=0.0)
+    s = make_solver(deck)
+    s.time = 0.0
+    mx, my = 2, 1
+    ky = 2 * PI * my / deck.ly
+    A = (0.3 - 0.4j, 0.25 + 0.1j)
+    V = np.zeros((3, deck.nx + 1, 2 * deck.nz + 1, deck.ny + 4), dtype=np.complex128)
+    for A_, iz in zip(A, (0, 1)):
+        V[1, mx, iz + deck.nz] = A_ * np.cos(ky * s.y)
+    s.set_V(V)
+    s.fill_all_ghosts()
+    p = s.pressure()[..., 2:deck.ny + 2]
+    yy = s.y[2:deck.ny + 2]
+    s2now = s.s2_of(0.0)
+    err, big = 0.0, 0.0
+    for A_, iz in zip(A, (0, 1)):
+        al, be = deck.alfa0 * mx, deck.beta0 * iz
+        B = 2j * (deck.S * al + s2now * be) * A_ / (al * al + be * be + ky * ky)
+        err = max(err, float(np.abs(p[mx, iz + deck.nz] - B * np.cos(ky * yy)).max()))
+        big = max(big, abs(B))
+    return err / big
+

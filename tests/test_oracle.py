@@ -248,3 +248,46 @@ def test_planted_mistake_is_caught(O):
     o = O.run_deck(d)
     _, Vr, _ = cplio.read_field(f"{GOLDEN}/reference/small.fld")
     assert cplio.compare_fields(Vr, o.V) > 1e-10
+
+
+def test_pressure_taylor_green(O):
+    """test_pressure (tests/test_pressure.f90:91-100): the two nonzero modes to
+    1e-4 (1.9e-5 at ny = 64), every other mode below 1e-10."""
+    err, rest = closed_forms.pressure_taylor_green(O.Oracle, read_input(deck_path("pressure")))
+    assert err <= 1e-4 and rest <= 1e-10, (err, rest)
+    assert err > 1e-7  # the fourth-order discretisation error is there: not a copy of the answer
+
+
+def test_pressure_shear_term(O):
+    """The S and S2 source terms on single modes against the continuous inverse."""
+    err = closed_forms.pressure_shear(O.Oracle, read_input(deck_path("pressure")))
+    assert err <= 1e-4, err
+
+
+def test_pressure_planted_sign_is_caught(O):
+    """Flipping the sign of S must break the shear check (the term is seen)."""
+    d = read_input(deck_path("pressure"))
+    make = lambda deck: O.Oracle(dataclasses.replace(deck, S=-1.0))
+    assert closed_forms.pressure_shear(make, d) > 0.5
+
+
+def test_column_energy_sums_to_the_energy(O):
+    """The column energies of outstats (src/hst_stats.f90:224-238): their sum
+    less the squared box means is 2 ly x the outstats energy; one mode of u at
+    (ix, iz) = (2, -3) with amplitude a gives |a|^2 ly in that column alone."""
+    d = read_input(deck_path("pressure"))
+    o = O.Oracle(d)
+    o.initial_field()
+    o.field(1)[0, d.nz, :] += 0.3 + 0.2 * np.sin(2 * np.pi * o.y / d.ly)
+    o.fill_all_ghosts(0.0)
+    E = o.column_energy()
+    st = o.outstats()
+    mean = (st[8] ** 2 + st[9] ** 2) / o.dyl.sum()
+    assert abs(E.sum() - mean - 2 * d.ly * st[0]) <= 1e-12 * E.sum()
+    o2 = O.Oracle(d)
+    a = 0.7 - 0.2j
+    o2.field(1)[2, -3 + d.nz, :] = a * np.sin(2 * np.pi * 3 * o2.y / d.ly)
+    E2 = o2.column_energy()
+    want = np.zeros_like(E2)
+    want[2, -3 + d.nz] = abs(a) ** 2 * d.ly
+    assert np.abs(E2 - want).max() <= 1e-13
