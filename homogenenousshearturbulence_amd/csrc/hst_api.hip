@@ -79,6 +79,14 @@ struct State {
   double *exz_acc = nullptr;             // column energies summed over the outstats calls
   int exz_n = 0;
   double mean_acc = 0.0;                 // summed squared box means of the (0,0) column
+  // physics hooks: the damping rates per column (rebuilt when the direction
+  // changes), the layer's body force rows and the rows inside the layer
+  double *d_gdamp = nullptr, *d_fy = nullptr;
+  int *d_inlayer = nullptr;
+  std::vector<int> inlayer;
+  double theta_last = 0.0;
+  bool theta_set = false;
+  double extra[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // the last outstats' layer and damping lines
   int64_t bytes_held = 0;
   bool prof = false;
   std::vector<Pending> pending;
@@ -200,6 +208,66 @@ void deck_shifts(const hst_deck &d, double lx, double lz, double t, double &sx, 
 
 static void shear_shifts(double t, double &sx, double &sz) { deck_shifts(S.deck, S.g.lx, S.g.lz, t, sx, sz); }
 
+// ------------------------------------------------------------------ physics hooks
+
+// stokes_active, damping_active, damping_soft_factor, damping_theta
+// (src/hst_stokes.f90:48-50, src/hst_damping.f90:45-67)
+static bool stokes_active() { return S.deck.sl_amplitude != 0.0; }
+static bool damping_active() {
+  return S.deck.damp_kpar > 0.0 && S.deck.damp_kperp > 0.0 && (S.deck.damp_hard || S.deck.damp_rate > 0.0);
+}
+static double damping_soft_factor() { return (damping_active() && !S.deck.damp_hard) ? 1.0 : 0.0; }
+static double damping_theta(double t) {
+  if (S.deck.damp_follow) return std::atan2(s2_of(t), S.deck.S);
+  return S.deck.damp_angle * 3.141592653589793 / 180.0;
+}
+
+// the switch-on factor of the layer (src/hst_stokes.f90:52-62)
+static double stokes_ramp(double t) {
+  const hst_deck &d = S.deck;
+  if (t < d.sl_start) return 0.0;
+  if (d.sl_ramp) {
+    const double tau = std::min((t - d.sl_start) / d.sl_period, 1.0);
+    return 3.0 * tau * tau - 2.0 * tau * tau * tau;
+  }
+  return 1.0;
+}
+
+static StokesParams stokes_params() {
+  return StokesParams{S.deck.sl_amplitude, S.deck.sl_period, S.deck.sl_delta, S.deck.sl_start, S.deck.ly, S.g.ni};
+}
+
+// damping_update (src/hst_damping.f90:78-115): the table for the direction at
+// t, rebuilt only when the direction changed since the last build
+static int damping_update(double t) {
+  if (!damping_active()) return 0;
+  const double th = damping_theta(t);
+  if (S.theta_set && th == S.theta_last) return 0;
+  S.theta_set = true;
+  S.theta_last = th;
+  const hst_deck &d = S.deck;
+  const DampParams p{d.damp_kpar, d.damp_kperp, d.damp_rate, d.damp_edge, d.damp_norm, d.damp_hard, d.damp_mean};
+  return run("damping_table", 0.0, [&] { return launch_damping_table(S.D, p, th, S.d_gdamp, S.stream); });
+}
+
+// stokes_force (src/hst_stokes.f90:90-101): the body force rows at t
+static bool stokes_body() { return stokes_active() && S.deck.sl_bodyforce; }
+static int stokes_force(double t) {
+  if (!stokes_body()) return 0;
+  return run("stokes_force", 0.0, [&] {
+    return launch_stokes_force(stokes_params(), S.D.ny, S.d_y, t, stokes_ramp(t), S.d_fy, S.stream);
+  });
+}
+
+// stokes_apply (src/hst_stokes.f90:103-118) at t, with the ghost rows of the column
+static int stokes_apply(double t) {
+  if (!stokes_active()) return 0;
+  const int whole = (!S.deck.sl_bodyforce || (t - S.deck.sl_start <= 2.0 * S.deck.sl_period)) ? 1 : 0;
+  return run("stokes_apply", 0.0, [&] {
+    return launch_stokes_apply(S.D, stokes_params(), S.d_y, t, stokes_ramp(t), whole, S.F[HST_V3], S.stream);
+  });
+}
+
 // ------------------------------------------------------------------ the closure's view of the state
 
 static StressHook g_hook = nullptr;
@@ -264,7 +332,10 @@ static int nonlinear_pass(int mode, bool want_cfl, double *cfl) {
 
 static double bytes_field() { return (double)S.D.ny * S.D.ncol * 16.0; }
 
-static int buildrhs(const double ode[3], double deltat, double time, int mode, bool shift, double dgx, double dgz) {
+// buildrhs_prepare and/or buildrhs (mode 0 prepare, 1/2 one group, 3 all);
+// mask: the hard damping mask fused into the stores (a whole substep only)
+static int buildrhs(const double ode[3], double deltat, double time, int mode, bool shift, double dgx, double dgz,
+                    bool mask = false) {
   RhsArgs a;
   a.ode0 = ode[0]; a.ode1 = ode[1]; a.ode2 = ode[2];
   a.deltat = deltat;
@@ -272,6 +343,16 @@ static int buildrhs(const double ode[3], double deltat, double time, int mode, b
   a.dgx = dgx; a.dgz = dgz;
   a.nonlinear = mode;
   a.shift = shift ? 1 : 0;
+  if (damping_active()) {
+    a.gdamp = S.d_gdamp;
+    a.gsoft = damping_soft_factor();
+    a.hard = (mask && S.deck.damp_hard) ? 1 : 0;
+  }
+  a.no_mean_vw = stokes_body() ? 1 : 0;
+  if (mode == 0 || mode == 3) {
+    if (int rc = stokes_force(time)) return rc;
+    if (stokes_body()) a.fy = S.d_fy;
+  }
   const void *P[6];
   for (int p = 0; p < 6; p++) P[p] = P_of(p);
   // prepare: u, v, w and oldrhs in, rhs and oldrhs out; a group adds its three
@@ -461,6 +542,8 @@ static void release() {
   for (double2 *p : S.XRo) if (p) (void)hipFree(p);
   if (S.PX) (void)hipFree(S.PX);
   if (S.exz_acc) (void)hipFree(S.exz_acc);
+  for (void *p : {(void *)S.d_gdamp, (void *)S.d_fy, (void *)S.d_inlayer})
+    if (p) (void)hipFree(p);
   if (S.PZ) (void)hipFree(S.PZ);
   S = State();
 }
@@ -548,6 +631,18 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
   }
   HIPCHK(hipMalloc(&S.exz_acc, (size_t)D.ncol * sizeof(double)));
   HIPCHK(hipMemset(S.exz_acc, 0, (size_t)D.ncol * sizeof(double)));
+  // physics hooks: gdamp (zero until the first damping_update), fy, and the
+  // rows inside |y - ly/2| < 8 delta (stokes_setup, src/hst_stokes.f90:76-86)
+  HIPCHK(hipMalloc(&S.d_gdamp, (size_t)D.ncol * sizeof(double)));
+  HIPCHK(hipMemset(S.d_gdamp, 0, (size_t)D.ncol * sizeof(double)));
+  HIPCHK(hipMalloc(&S.d_fy, (size_t)D.NYg * sizeof(double)));
+  HIPCHK(hipMemset(S.d_fy, 0, (size_t)D.NYg * sizeof(double)));
+  S.inlayer.assign(D.ny, 0);
+  if (stokes_active())
+    for (int iy = 0; iy < D.ny; iy++)
+      S.inlayer[iy] = std::fabs(S.y[iy + 2] - 0.5 * S.deck.ly) < 8.0 * S.deck.sl_delta ? 1 : 0;
+  HIPCHK(hipMalloc(&S.d_inlayer, (size_t)D.ny * sizeof(int)));
+  HIPCHK(hipMemcpy(S.d_inlayer, S.inlayer.data(), (size_t)D.ny * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&S.PX, (size_t)D.ny * D.NXm * sizeof(double2)));
   HIPCHK(hipMalloc(&S.PZ, (size_t)D.ny * D.NZm * sizeof(double2)));
   // transform chunk: rows of the physical-space buffers Hv (3 fields) and Hp (6)
@@ -577,7 +672,7 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
   HIPCHK(hipMemcpy(S.d_dyl, S.dyl.data(), S.dyl.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&S.d_f0, S.f0.rows.size() * sizeof(double)));
   HIPCHK(hipMemcpy(S.d_f0, S.f0.rows.data(), S.f0.rows.size() * sizeof(double), hipMemcpyHostToDevice));
-  S.part_cap = std::max<size_t>((size_t)D.NYg * D.nzd, (size_t)((D.ncol + 255) / 256) * 9 + 64);
+  S.part_cap = std::max<size_t>((size_t)D.NYg * D.nzd, (size_t)((D.ncol + 255) / 256) * NSTAT + 64);
   HIPCHK(hipMalloc(&S.d_part, S.part_cap * sizeof(double)));
   if (make_fft_tables(D, S.mixed, &S.tables)) return fail("twiddle tables");
   S.bytes_held = held;
@@ -676,7 +771,24 @@ int hst_gpu_product_to_field(int p, int field) {
 
 int hst_gpu_buildrhs_prepare(const double ode[3], double deltat, double time) {
   if (int rc = check_live()) return rc;
+  // the substep's damping_update precedes buildrhs_prepare (src/hst_equations.f90:64)
+  if (int rc = damping_update(time)) return rc;
   return buildrhs(ode, deltat, time, 0, false, 0.0, 0.0);
+}
+
+int hst_gpu_damping_mask(void) {
+  if (int rc = check_live()) return rc;
+  if (!(damping_active() && S.deck.damp_hard)) return 0;
+  if (!S.theta_set) return fail("damping_mask: no damping table yet (hst_gpu_buildrhs_prepare builds it)");
+  return run("damping_mask", 4.0 * bytes_field(), [&] {
+    return launch_damping_mask(S.D, S.mixed, S.d_gdamp, S.F[HST_RHS1], S.F[HST_RHS2], S.F[HST_OLD1], S.F[HST_OLD2],
+                               S.stream);
+  });
+}
+
+int hst_gpu_stokes_apply(double time) {
+  if (int rc = check_live()) return rc;
+  return stokes_apply(time);
 }
 
 int hst_gpu_buildrhs(const double ode[3], int g) {
@@ -732,11 +844,13 @@ int hst_gpu_timestep(double deltat, double *time, double *cfl) {
     const bool fuse_shift = !S.deck.exact_shift;
     const double dgx = S.deck.S * dt_sub, dgz = s2_integral(t, t + dt_sub);
     if ((rc = join_side())) return rc;  // the previous substep's oldrhs shift
-    rc = buildrhs(ode, deltat, t, S.deck.linear ? 0 : 3, fuse_shift, dgx, dgz);
+    if ((rc = damping_update(t))) return rc;
+    rc = buildrhs(ode, deltat, t, S.deck.linear ? 0 : 3, fuse_shift, dgx, dgz, true);
     if (rc) return rc;
     if (S.deck.exact_shift && (rc = exact_shift_split(dt_sub, t))) return rc;
     t = t + dt_sub;
     if ((rc = linsolve(ode[0] / deltat, t))) return rc;
+    if ((rc = stokes_apply(t))) return rc;
   }
   *time = t;
   if (cfl) *cfl = c;
@@ -745,39 +859,63 @@ int hst_gpu_timestep(double deltat, double *time, double *cfl) {
 
 int hst_gpu_outstats(double time, double out[10]) {
   if (int rc = check_live()) return rc;
+  if (!out) return fail("outstats: out is NULL");
+  // the damping table as set up at the start, or as last rebuilt by a substep
+  if (!S.theta_set)
+    if (int rc = damping_update(time)) return rc;
+  StatsArgs a;
+  if (damping_active()) {
+    a.gdamp = S.d_gdamp;
+    a.gsoft = damping_soft_factor();
+  }
+  if (stokes_active()) {
+    a.inlayer = S.d_inlayer;
+    a.inc00 = 0;  // the (0,0) column is the imposed layer: mean flow
+  }
   int nb = 0;
   int rc = run("stats", 3.0 * bytes_field(), [&] {
-    return launch_stats(S.D, S.F[1], S.F[2], S.F[3], S.d_dyl, S.d_part, &nb, S.exz_acc, S.stream);
+    return launch_stats(S.D, a, S.F[1], S.F[2], S.F[3], S.d_dyl, S.d_part, &nb, S.exz_acc, S.stream);
   });
   if (rc) return rc;
-  std::vector<double> h((size_t)nb * 9);
+  std::vector<double> h((size_t)nb * NSTAT);
   HIPCHK(hipMemcpyAsync(h.data(), S.d_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, S.stream));
   HIPCHK(hipStreamSynchronize(S.stream));
-  double s[9] = {0};
+  double s[NSTAT] = {0};
   for (int b = 0; b < nb; b++)
-    for (int k = 0; k < 9; k++) s[k] += h[(size_t)b * 9 + k];
+    for (int k = 0; k < NSTAT; k++) s[k] += h[(size_t)b * NSTAT + k];
   double sx, sz;
   shear_shifts(time, sx, sz);
-  double eps = s[6];
+  double eps = s[6], e_in = s[11], e_out = s[12];
   for (int c = 1; c <= 3; c++) {
     int np = 0;
     rc = run("dy_stats", 3.0 * bytes_field(), [&] {
       return launch_d0solve(S.D, 1, 2, S.F[c], nullptr, nullptr, S.XR[0], S.d_f0, S.fc, S.d_der, S.d_dyl, sx, sz,
-                            S.d_part, &np, S.stream);
+                            S.d_part, &np, S.stream, a.inlayer, a.inc00 ? 0 : 1);
     });
     if (rc) return rc;
-    std::vector<double> g((size_t)np);
+    std::vector<double> g((size_t)np * 3);
     HIPCHK(hipMemcpyAsync(g.data(), S.d_part, g.size() * sizeof(double), hipMemcpyDeviceToHost, S.stream));
     HIPCHK(hipStreamSynchronize(S.stream));
-    for (double v : g) eps += v;
+    double grad = 0.0, g_in = 0.0, g_out = 0.0;
+    for (int b = 0; b < np; b++) {
+      grad += g[(size_t)b * 3];
+      g_in += g[(size_t)b * 3 + 1];
+      g_out += g[(size_t)b * 3 + 2];
+    }
+    eps += grad;
+    e_in += g_in;
+    e_out += g_out;
   }
   double uu = s[0], vv = s[1], ww = s[2], uv = s[3], vw = s[4], uw = s[5], mfx = s[7], mfz = s[8];
-  double ly_w = 0.0;
-  for (double d : S.dyl) ly_w += d;
-  uu = uu - mfx * mfx / ly_w;
-  ww = ww - mfz * mfz / ly_w;
-  uw = uw - mfx * mfz / ly_w;
-  S.mean_acc += (mfx * mfx + mfz * mfz) / ly_w;
+  if (a.inc00) {
+    // squared box means off the (0,0) column's raw sums
+    double ly_w = 0.0;
+    for (double d : S.dyl) ly_w += d;
+    uu = uu - mfx * mfx / ly_w;
+    ww = ww - mfz * mfz / ly_w;
+    uw = uw - mfx * mfz / ly_w;
+    S.mean_acc += (mfx * mfx + mfz * mfz) / ly_w;
+  }
   S.exz_n += 1;
   const double q2 = uu + vv + ww, inv_ly = 1.0 / S.deck.ly;
   out[0] = 0.5 * q2 * inv_ly;
@@ -790,6 +928,29 @@ int hst_gpu_outstats(double time, double out[10]) {
   out[7] = uw * inv_ly;
   out[8] = mfx;
   out[9] = mfz;
+  // stokes_runtime.dat and damping_runtime.dat (src/hst_stats.f90:270-284)
+  for (double &v : S.extra) v = 0.0;
+  if (stokes_active()) {
+    double l_in = 0.0, l_out = 0.0;
+    for (int iy = 0; iy < S.D.ny; iy++) (S.inlayer[iy] == 1 ? l_in : l_out) += S.dyl[iy];
+    S.extra[0] = 0.5 * s[10] / l_out;
+    S.extra[1] = 0.5 * s[9] / l_in;
+    S.extra[2] = S.g.ni * e_out / l_out;
+    S.extra[3] = S.g.ni * e_in / l_in;
+  }
+  if (damping_active()) {
+    S.extra[4] = damping_theta(time) * 180.0 / 3.141592653589793;
+    S.extra[5] = s[15];
+    S.extra[6] = 0.5 * s[13];
+    S.extra[7] = 0.5 * s[14];
+  }
+  return 0;
+}
+
+int hst_gpu_outstats_extra(double out[8]) {
+  if (int rc = check_live()) return rc;
+  if (!out) return fail("outstats_extra: out is NULL");
+  for (int k = 0; k < 8; k++) out[k] = S.extra[k];
   return 0;
 }
 

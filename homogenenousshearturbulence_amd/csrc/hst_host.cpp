@@ -66,8 +66,15 @@ static void solve5(double M[5][5], double b[5], double x[5]) {
 std::string check_deck(const hst_deck &d) {
   if (d.nx < 1 || d.nz < 1) return "nx and nz must be positive";
   if (d.ny < 8) return "ny must be at least 8";
-  if (d.sl_amplitude != 0.0) return "the Stokes layer (sl_amplitude) is not part of this library";
-  if (d.damp_kpar != 0.0 || d.damp_kperp != 0.0) return "large-scale damping is not part of this library";
+  // the checks of read_input (src/hst_input.f90:67-82)
+  if (d.sl_amplitude != 0.0 && (d.s2_amplitude != 0.0 || d.s2_mean != 0.0))
+    return "the Stokes layer and the spanwise shear S2 cannot both be active";
+  if (d.sl_amplitude != 0.0 && !(d.sl_period > 0.0)) return "sl_period must be positive";
+  if (d.sl_amplitude != 0.0 && !(d.sl_delta > 0.0)) return "sl_delta must be positive";
+  if ((d.damp_kpar > 0.0) != (d.damp_kperp > 0.0))
+    return "damp_kpar and damp_kperp must both be positive (damping on) or both zero (off)";
+  if (d.damp_kpar > 0.0 && (d.damp_norm < 1.0 || d.damp_edge < 0.0 || d.damp_edge >= 1.0 || d.damp_rate < 0.0))
+    return "damping needs damp_norm >= 1, 0 <= damp_edge < 1, damp_rate >= 0";
   if (!(d.re > 0.0)) return "re must be positive";
   if (!(d.alfa0 > 0.0) || !(d.beta0 > 0.0) || !(d.ly > 0.0)) return "alfa0, beta0 and ly must be positive";
   if (d.initial_condition == 1) {

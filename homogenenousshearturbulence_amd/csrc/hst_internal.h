@@ -79,6 +79,12 @@ double bytes_zfft(const Dims &D, int mixed, int np, int nrows);
 struct RhsArgs {
   double ode0, ode1, ode2, deltat, s2now, dgx, dgz;
   int nonlinear, shift;
+  // the physics hooks (all off: gdamp = fy = null, hard = no_mean_vw = 0)
+  const double *gdamp = nullptr;  // per-column damping rates [ncol]
+  double gsoft = 0.0;             // 1: the explicit damping term, 0: none or the hard mask
+  int hard = 0;                   // the masked columns' rhs and oldrhs stored as zero
+  int no_mean_vw = 0;             // the (0,0) column drops the vw divergence (layer with body force)
+  const double *fy = nullptr;     // the layer's body force at the rows -2..ny+1
 };
 // oldrhs (O1, O2) is FP32 (float2) in the mixed form, FP64 otherwise.
 int launch_phase4(const Dims &D, int mixed, double2 *R1, double2 *R2, void *O1, void *O2, const double *y,
@@ -102,10 +108,13 @@ int launch_penta_factor(const Dims &D, int kind, double lambda, const double *de
 // D0-type solves with the shared factorisation.  rhs_mode 0: D0 x = src;
 // 1: D0 x = D1 src (src ghost rows filled).  out_mode 0: x into dst;
 // 1: u, w recovered from x = dv/dy and eta (dst = V1 holding eta, dst3 = V3);
-// 2: sum of w |x|^2 into part (one double per block).
+// 2: sums of w |x|^2 into part, three doubles per block: all rows, the rows
+// with inlayer set, the others (inlayer may be null: every row outside); the
+// (0,0) column is left out when skip00 is set.
 int launch_d0solve(const Dims &D, int rhs_mode, int out_mode, const double2 *src, double2 *dst, double2 *dst3,
                    double2 *XR, const double *f0rows, const D0Const &fc, const double *der, const double *dyl,
-                   double sx, double sz, double *part, int *n_part, hipStream_t s);
+                   double sx, double sz, double *part, int *n_part, hipStream_t s, const int *inlayer = nullptr,
+                   int skip00 = 0);
 
 // The exact shift of q[0..nq) (rhs1, rhs2, oldrhs1, oldrhs2): D0 solve at the
 // old shifts, node phase, ghosts at the new shifts, D0 re-weight.
@@ -124,10 +133,43 @@ int launch_widen(const float2 *src, double2 *dst, size_t n, hipStream_t s);
 // Ghost rows of nf fields at the shifts (sx, sz).
 int launch_ghosts(const Dims &D, int nf, double2 *const *f, double sx, double sz, hipStream_t s);
 
-// First reduction of the statistics: per block 9 sums (uu vv ww uv vw uw eps mfx mfz).
+// First reduction of the statistics: per block NSTAT sums, in the order
+// uu vv ww uv vw uw eps mfx mfz q_in q_out e_in e_out ed pd ncol_damped.
 // exz_acc (may be null): the column energies sum_y w |u|^2 are added to it.
-int launch_stats(const Dims &D, const double2 *V1, const double2 *V2, const double2 *V3, const double *dyl,
-                 double *part, int *n_part, double *exz_acc, hipStream_t s);
+constexpr int NSTAT = 16;
+struct StatsArgs {
+  const double *gdamp = nullptr;  // damping rates (null: none)
+  double gsoft = 0.0;
+  const int *inlayer = nullptr;   // rows inside the Stokes layer (null: none)
+  int inc00 = 1;                  // the (0,0) column enters uu, ww, uw
+};
+int launch_stats(const Dims &D, const StatsArgs &a, const double2 *V1, const double2 *V2, const double2 *V3,
+                 const double *dyl, double *part, int *n_part, double *exz_acc, hipStream_t s);
+
+// The Stokes layer (src/hst_stokes.f90): the profile and body force as
+// closed forms of y and t; ramp: the factor of the switch-on at t.
+struct StokesParams {
+  double amp, period, delta, start, ly, ni;
+};
+// fy[row] for the rows -2..ny+1 (ghosts: the periodic copies of the rows
+// ny-2, ny-1, 0, 1) at time t (stokes_force, :90-101).
+int launch_stokes_force(const StokesParams &p, int ny, const double *y, double t, double ramp, double *fy,
+                        hipStream_t s);
+// stokes_apply (:103-118): the (0,0) mode of w set to W(y, t) on every row
+// (whole) or on rows 0 and ny-1, then its four ghost rows (phase 1).
+int launch_stokes_apply(const Dims &D, const StokesParams &p, const double *y, double t, double ramp, int whole,
+                        double2 *V3, hipStream_t s);
+
+// The damping rates g(kx, kz) for the direction theta (damping_update,
+// src/hst_damping.f90:78-115).
+struct DampParams {
+  double kpar, kperp, rate, edge, norm;
+  int hard, mean;
+};
+int launch_damping_table(const Dims &D, const DampParams &p, double theta, double *gdamp, hipStream_t s);
+// damping_mask (:119-133): rhs and oldrhs of the columns with g > 0 set to zero.
+int launch_damping_mask(const Dims &D, int mixed, const double *gdamp, double2 *R1, double2 *R2, void *O1, void *O2,
+                        hipStream_t s);
 // The pressure equation's right-hand side from the six product spectra and v
 // into R (interior rows), and the (0,0) column of p from it.
 int launch_pressure_rhs(const Dims &D, int mixed, const void *P0, const double2 *V2, double2 *R, const double *der,

@@ -74,6 +74,8 @@ __global__ void __launch_bounds__(256)
   const double kk = al * al + be * be;
   const bool m00 = (ix == 0 && iz == 0);
   const double gam = a.s2now * (ix * D.alfa0) - D.S * (iz * D.beta0);
+  const double gd = a.gdamp ? a.gsoft * a.gdamp[col] : 0.0;
+  const bool masked = a.hard && a.gdamp && a.gdamp[col] > 0.0;
   const int iy0 = blockIdx.y * rch, iy1 = min(iy0 + rch, D.ny);
   const size_t nc = D.ncol;
   // windows: w[k] holds row iy - 2 + k
@@ -122,12 +124,18 @@ __global__ void __launch_bounds__(256)
       impl = impl + (D.ni * biharm) * v[j];
     }
     C o2old = ld(O2 + c0), o1old = ld(O1 + c0);
-    // ODE(1) unkn / deltat + impl - ODE(3) oldrhs (the damping term is zero)
+    // ODE(1) unkn / deltat + impl - ODE(3) oldrhs - ODE(2) g unkn, g the
+    // column's explicit damping rate
     r2 = a.ode0 * unkn;
     r2 = C{r2.x / a.deltat, r2.y / a.deltat} + impl - a.ode2 * o2old;
     o2 = C{0.0, 0.0};
+    if (gd != 0.0) {
+      r2 = r2 - (a.ode1 * gd) * unkn;
+      o2 = C{-gd * unkn.x, -gd * unkn.y};
+    }
     // eta equation (:118-137), the (0,0) column packs (u, w)
     C unk1{0, 0}, imp1{0, 0}, expl{0, 0};
+    double force = 0.0;
 #pragma unroll
     for (int j = 0; j < 5; j++) {
       const double d0 = DER(der, iy, 0, j - 2), d2 = DER(der, iy, 2, j - 2);
@@ -136,6 +144,7 @@ __global__ void __launch_bounds__(256)
         const C f{u[j].x, w[j].x};
         unk1 = unk1 + d0 * f;
         imp1 = imp1 + (D.ni * d2) * f;
+        if (a.fy) force = force + d0 * a.fy[iy + j];  // the layer's body force on the mean w, D0-weighted
       } else {
         const C f = imul(be, u[j]) - imul(al, w[j]);
         unk1 = unk1 + d0 * f;
@@ -143,6 +152,8 @@ __global__ void __launch_bounds__(256)
       }
       expl = expl + imul(gam * d0, v[j]);
     }
+    if (m00 && a.fy) expl = expl + C{0.0, force};
+    if (gd != 0.0) expl = expl - gd * unk1;
     r1 = (a.ode0 * unk1);
     r1 = C{r1.x / a.deltat, r1.y / a.deltat} + imp1 - a.ode2 * o1old + a.ode1 * expl;
     o1 = expl;
@@ -170,7 +181,10 @@ __global__ void __launch_bounds__(256)
         else if (q == 1) { ex = kkp * d1; }
         else if (q == 2) { rw = -imul(bep, d0); ex = d1 * (-(bep * bep)); }
         else if (q == 3) { ru = -d1; ex = imul(alp, d2) + imul(alp * kkp, d0); }
-        else if (q == 4) { rw = -d1; ex = imul(bep, d2) + imul(bep * kkp, d0); }
+        else if (q == 4) {
+          rw = (a.no_mean_vw && m00) ? Cx<PR>{0, 0} : -d1;
+          ex = imul(bep, d2) + imul(bep * kkp, d0);
+        }
         else { ru = -imul(bep, d0); rw = -imul(alp, d0); ex = d1 * (-((PR)2 * alp * bep)); }
         const C exd{(double)ex.x, (double)ex.y};
         r2 = r2 + a.ode1 * exd;
@@ -189,6 +203,9 @@ __global__ void __launch_bounds__(256)
       // shear_shift, default advection (src/hst_equations.f90:245-257)
       const C f = cexpi(-((D.alfa0 * ix) * a.dgx + (D.beta0 * iz) * a.dgz) * y[iy + 2]);
       r1 = r1 * f; r2 = r2 * f; o1 = o1 * f; o2 = o2 * f;
+    }
+    if (masked) {  // damping_mask (src/hst_damping.f90:119-133), before the shift
+      r1 = C{0, 0}; r2 = C{0, 0}; o1 = C{0, 0}; o2 = C{0, 0};
     }
     st(R1 + c0, r1); st(R2 + c0, r2); st(O1 + c0, o1); st(O2 + c0, o2);
   }
@@ -626,7 +643,8 @@ template <int RHS, int OUT>
 __global__ void __launch_bounds__(256)
     k_d0solve(Dims D, D0Const fc, const double *__restrict__ f0, const double *__restrict__ der,
               const double *__restrict__ dyl, const double2 *__restrict__ src, double2 *__restrict__ dst,
-              double2 *__restrict__ dst3, double2 *__restrict__ XR, double sx, double sz, double *__restrict__ part) {
+              double2 *__restrict__ dst3, double2 *__restrict__ XR, double sx, double sz, double *__restrict__ part,
+              const int *__restrict__ inlayer, int skip00) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   const bool act = col < D.ncol;
   const int cl = act ? col : 0;
@@ -634,7 +652,8 @@ __global__ void __launch_bounds__(256)
   col_ix_iz(D, cl, ix, iz);
   const size_t nc = D.ncol;
   const int m = D.ny;
-  double acc = 0.0;
+  double acc = 0.0, ain = 0.0, aout = 0.0;
+  const bool skip = skip00 && ix == 0 && iz == 0;
   if (act) {
     C x1, x2, sx0, sx1, r25, r27;
     d0_forward<RHS>(D, f0, der, src, XR, col, x1, x2, sx0, sx1, r25, r27);
@@ -659,9 +678,12 @@ __global__ void __launch_bounds__(256)
           st(dst + o, C{t.x / kk, t.y / kk});
           st(dst3 + o, C{ww.x / kk, ww.y / kk});
         }
-      } else {
+      } else if (!skip) {
         const double wgt = (ix == 0 ? dyl[i] : 2.0 * dyl[i]);
-        acc += wgt * (x.x * x.x + x.y * x.y);
+        const double q = wgt * (x.x * x.x + x.y * x.y);
+        acc += q;
+        if (inlayer && inlayer[i]) ain += q;
+        else aout += q;
       }
     };
     auto eta_at = [&](int i) -> C {
@@ -684,21 +706,29 @@ __global__ void __launch_bounds__(256)
         });
   }
   if constexpr (OUT == 2) {
-    __shared__ double red[4];
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __shared__ double red[3][4];
+    double v[3] = {acc, ain, aout};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
+      if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = v[k];
+    }
     __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x < 3) {
+      const int k = threadIdx.x;
+      part[(size_t)blockIdx.x * 3 + k] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+    }
   }
 }
 
 int launch_d0solve(const Dims &D, int rhs_mode, int out_mode, const double2 *src, double2 *dst, double2 *dst3,
                    double2 *XR, const double *f0rows, const D0Const &fc, const double *der, const double *dyl,
-                   double sx, double sz, double *part, int *n_part, hipStream_t s) {
+                   double sx, double sz, double *part, int *n_part, hipStream_t s, const int *inlayer, int skip00) {
   dim3 grid((D.ncol + 255) / 256);
   if (n_part) *n_part = grid.x;
-#define L(R, O) \
-  hipLaunchKernelGGL((k_d0solve<R, O>), grid, dim3(256), 0, s, D, fc, f0rows, der, dyl, src, dst, dst3, XR, sx, sz, part)
+#define L(R, O)                                                                                                 \
+  hipLaunchKernelGGL((k_d0solve<R, O>), grid, dim3(256), 0, s, D, fc, f0rows, der, dyl, src, dst, dst3, XR, sx, sz, \
+                     part, inlayer, skip00)
   if (rhs_mode == 0) {
     if (out_mode == 0) L(0, 0); else if (out_mode == 1) L(0, 1); else L(0, 2);
   } else {
@@ -861,36 +891,53 @@ int launch_ghosts(const Dims &D, int nf, double2 *const *f, double sx, double sz
 
 // ------------------------------------------------------------------ statistics
 
-// The first reduction of outstats (src/hst_stats.f90:153-198) without the
-// Stokes layer and damping: per block the sums uu vv ww uv vw uw eps mfx mfz.
+// The first reduction of outstats (src/hst_stats.f90:153-198): per block the
+// NSTAT sums uu vv ww uv vw uw eps mfx mfz q_in q_out e_in e_out ed pd
+// ncol_damped; the column energies of the spectrum on the way (:224-238).
 __global__ void __launch_bounds__(256)
-    k_stats(Dims D, const double2 *__restrict__ V1, const double2 *__restrict__ V2, const double2 *__restrict__ V3,
-            const double *__restrict__ dyl, double *__restrict__ part, double *__restrict__ exz) {
+    k_stats(Dims D, StatsArgs a, const double2 *__restrict__ V1, const double2 *__restrict__ V2,
+            const double2 *__restrict__ V3, const double *__restrict__ dyl, double *__restrict__ part,
+            double *__restrict__ exz) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double s[NSTAT];
+#pragma unroll
+  for (int k = 0; k < NSTAT; k++) s[k] = 0.0;
   if (col < D.ncol) {
     int ix, iz;
     col_ix_iz(D, col, ix, iz);
     const double al = D.alfa0 * ix, be = D.beta0 * iz, kk = al * al + be * be;
     const size_t nc = D.ncol;
-    // the column energy sum_y w |u|^2 of the spectrum (src/hst_stats.f90:225-236), every column
+    const double gd = a.gdamp ? a.gdamp[col] : 0.0;
+    if (gd > 0.0) s[15] = 1.0;
+    // the column energy sum_y w |u|^2 of the spectrum, every column
     double qc = 0.0;
     for (int iy = 0; iy < D.ny; iy++) {
       const size_t o = (size_t)(iy + 2) * nc + col;
       const C cu = ld(V1 + o), cv = ld(V2 + o), cw = ld(V3 + o);
       const double dy = dyl[iy];
-      qc = qc + (ix == 0 ? dy : 2.0 * dy) * (cu.x * cu.x + cu.y * cu.y + cv.x * cv.x + cv.y * cv.y + cw.x * cw.x +
-                                             cw.y * cw.y);
+      const double uu = cu.x * cu.x + cu.y * cu.y, vv = cv.x * cv.x + cv.y * cv.y, wv = cw.x * cw.x + cw.y * cw.y;
+      qc = qc + (ix == 0 ? dy : 2.0 * dy) * (uu + vv + wv);
       if (ix == 0 && iz == 0) {
         s[7] += cu.x * dy;
         s[8] += cw.x * dy;
-        s[0] += dy * (cu.x * cu.x + cu.y * cu.y);
-        s[2] += dy * (cw.x * cw.x + cw.y * cw.y);
-        s[5] += dy * (cu.x * cw.x + cu.y * cw.y);
+        if (gd > 0.0) {  // damped mean profiles (damp_mean)
+          const double q = dy * (uu + wv);
+          s[13] += q;
+          s[14] += 2.0 * a.gsoft * gd * q;
+        }
+        if (a.inc00) {
+          s[0] += dy * uu;
+          s[2] += dy * wv;
+          s[5] += dy * (cu.x * cw.x + cu.y * cw.y);
+        }
         continue;
       }
       const double w = ix == 0 ? dy : 2.0 * dy;
-      const double uu = cu.x * cu.x + cu.y * cu.y, vv = cv.x * cv.x + cv.y * cv.y, wv = cw.x * cw.x + cw.y * cw.y;
+      const double q = w * (uu + vv + wv);
+      if (gd > 0.0) {
+        s[13] += q;
+        s[14] += 2.0 * a.gsoft * gd * q;
+      }
       s[0] += w * uu;
       s[1] += w * vv;
       s[2] += w * wv;
@@ -898,28 +945,149 @@ __global__ void __launch_bounds__(256)
       s[4] += w * (cw.x * cv.x + cw.y * cv.y);
       s[5] += w * (cu.x * cw.x + cu.y * cw.y);
       s[6] += w * kk * (uu + vv + wv);
+      if (a.inlayer && a.inlayer[iy]) {
+        s[9] += q;
+        s[11] += w * kk * (uu + vv + wv);
+      } else {
+        s[10] += q;
+        s[12] += w * kk * (uu + vv + wv);
+      }
     }
     if (exz) exz[col] += qc;
   }
-  __shared__ double red[4][9];
+  __shared__ double red[4][NSTAT];
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
+  for (int k = 0; k < NSTAT; k++) {
     double v = s[k];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = v;
   }
   __syncthreads();
-  if (threadIdx.x < 9) {
+  if (threadIdx.x < NSTAT) {
     const int k = threadIdx.x;
-    part[(size_t)blockIdx.x * 9 + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+    part[(size_t)blockIdx.x * NSTAT + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
   }
 }
 
-int launch_stats(const Dims &D, const double2 *V1, const double2 *V2, const double2 *V3, const double *dyl,
-                 double *part, int *n_part, double *exz_acc, hipStream_t s) {
+int launch_stats(const Dims &D, const StatsArgs &a, const double2 *V1, const double2 *V2, const double2 *V3,
+                 const double *dyl, double *part, int *n_part, double *exz_acc, hipStream_t s) {
   dim3 grid((D.ncol + 255) / 256);
   if (n_part) *n_part = grid.x;
-  hipLaunchKernelGGL(k_stats, grid, dim3(256), 0, s, D, V1, V2, V3, dyl, part, exz_acc);
+  hipLaunchKernelGGL(k_stats, grid, dim3(256), 0, s, D, a, V1, V2, V3, dyl, part, exz_acc);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ physics hooks
+
+// The layer profile W and body force f (src/hst_stokes.f90:63-88) at y, t.
+__device__ __forceinline__ double stokes_s(const StokesParams &p, double yy) {
+  const double e = (yy - 0.5 * p.ly) / p.delta;
+  return sqrt(e * e + 0.1 * 0.1);  // eps = 0.1: the smoothed |eta| / delta
+}
+
+__global__ void k_stokes_force(StokesParams p, int ny, const double *__restrict__ y, double t, double ramp,
+                               double *__restrict__ fy) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;  // row + 2
+  if (r >= ny + 4) return;
+  const int iy = ((r - 2) % ny + ny) % ny;              // ghosts: the periodic copies
+  const double yy = y[iy + 2];
+  const double omega = 6.283185307179586 / p.period;
+  const double eta = yy - 0.5 * p.ly;
+  const double sv = stokes_s(p, yy);
+  const double s1 = (eta / (p.delta * p.delta)) / sv;
+  const double s2 = ((0.1 * 0.1) / (p.delta * p.delta)) / (sv * sv * sv);
+  const double th = omega * (t - p.start) - sv;
+  double sn, cs;
+  sincos(th, &sn, &cs);
+  fy[r] = ramp * p.amp * exp(-sv) * ((2.0 * p.ni * s1 * s1 - omega) * sn - p.ni * s2 * (sn - cs));
+}
+
+int launch_stokes_force(const StokesParams &p, int ny, const double *y, double t, double ramp, double *fy,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_stokes_force, dim3((ny + 4 + 255) / 256), dim3(256), 0, s, p, ny, y, t, ramp, fy);
+  return hipGetLastError();
+}
+
+// One block: the interior rows of the (0,0) column of w, then its ghosts.
+__global__ void __launch_bounds__(256)
+    k_stokes_apply(Dims D, StokesParams p, const double *__restrict__ y, double t, double ramp, int whole,
+                   double2 *__restrict__ V3) {
+  const size_t c00 = (size_t)D.nz, nc = D.ncol;
+  const int ny = D.ny;
+  const double omega = 6.283185307179586 / p.period;
+  for (int iy = threadIdx.x; iy < ny; iy += blockDim.x) {
+    if (whole || iy == 0 || iy == ny - 1) {
+      const double sv = stokes_s(p, y[iy + 2]);
+      V3[(size_t)(iy + 2) * nc + c00] = make_double2(ramp * p.amp * exp(-sv) * cos(omega * (t - p.start) - sv), 0.0);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {  // rows -2, -1, ny, ny+1 from ny-2, ny-1, 0, 1 (the phase of the (0,0) column is 1)
+    const int g = threadIdx.x < 2 ? (int)threadIdx.x - 2 : ny + (int)threadIdx.x - 2;
+    const int src = threadIdx.x < 2 ? ny + g : g - ny;
+    V3[(size_t)(g + 2) * nc + c00] = V3[(size_t)(src + 2) * nc + c00];
+  }
+}
+
+int launch_stokes_apply(const Dims &D, const StokesParams &p, const double *y, double t, double ramp, int whole,
+                        double2 *V3, hipStream_t s) {
+  hipLaunchKernelGGL(k_stokes_apply, dim3(1), dim3(256), 0, s, D, p, y, t, ramp, whole, V3);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256)
+    k_damping_table(Dims D, DampParams p, double c, double sn, double *__restrict__ gdamp) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= D.ncol) return;
+  int ix, iz;
+  col_ix_iz(D, col, ix, iz);
+  const double kx = D.alfa0 * ix, kz = D.beta0 * iz;
+  const double kpar = kx * c + kz * sn, kperp = -kx * sn + kz * c;
+  const double r = pow(pow(fabs(kpar) / p.kpar, p.norm) + pow(fabs(kperp) / p.kperp, p.norm), 1.0 / p.norm);
+  double g = 0.0;
+  if (r < 1.0 - 1.0e-6) {
+    if (p.hard) {
+      g = 1.0;
+    } else if (r <= 1.0 - p.edge) {
+      g = p.rate;
+    } else {
+      const double x = (1.0 - r) / p.edge;
+      g = p.rate * x * x * (3.0 - 2.0 * x);
+    }
+  }
+  if (ix == 0 && iz == 0 && !p.mean) g = 0.0;
+  gdamp[col] = g;
+}
+
+int launch_damping_table(const Dims &D, const DampParams &p, double theta, double *gdamp, hipStream_t s) {
+  hipLaunchKernelGGL(k_damping_table, dim3((D.ncol + 255) / 256), dim3(256), 0, s, D, p, cos(theta), sin(theta),
+                     gdamp);
+  return hipGetLastError();
+}
+
+template <typename OT>
+__global__ void __launch_bounds__(256)
+    k_damping_mask(Dims D, const double *__restrict__ gdamp, double2 *R1, double2 *R2, OT *O1, OT *O2) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= D.ncol || !(gdamp[col] > 0.0)) return;
+  const size_t nc = D.ncol;
+  for (int iy = blockIdx.y; iy < D.ny; iy += gridDim.y) {
+    const size_t o = (size_t)(iy + 2) * nc + col;
+    R1[o] = make_double2(0.0, 0.0);
+    R2[o] = make_double2(0.0, 0.0);
+    O1[o] = OT{0, 0};
+    O2[o] = OT{0, 0};
+  }
+}
+
+int launch_damping_mask(const Dims &D, int mixed, const double *gdamp, double2 *R1, double2 *R2, void *O1, void *O2,
+                        hipStream_t s) {
+  dim3 grid((D.ncol + 255) / 256, std::min(D.ny, 64));
+  if (mixed)
+    hipLaunchKernelGGL(k_damping_mask<float2>, grid, dim3(256), 0, s, D, gdamp, R1, R2, (float2 *)O1, (float2 *)O2);
+  else
+    hipLaunchKernelGGL(k_damping_mask<double2>, grid, dim3(256), 0, s, D, gdamp, R1, R2, (double2 *)O1,
+                       (double2 *)O2);
   return hipGetLastError();
 }
 

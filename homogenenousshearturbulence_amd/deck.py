@@ -34,6 +34,11 @@ class hst_deck(ctypes.Structure):
         ("tg_amplitude", ctypes.c_double),
         ("sl_amplitude", ctypes.c_double),
         ("damp_kpar", ctypes.c_double), ("damp_kperp", ctypes.c_double),
+        ("sl_period", ctypes.c_double), ("sl_delta", ctypes.c_double), ("sl_start", ctypes.c_double),
+        ("damp_rate", ctypes.c_double), ("damp_edge", ctypes.c_double), ("damp_norm", ctypes.c_double),
+        ("damp_angle", ctypes.c_double),
+        ("sl_bodyforce", ctypes.c_int32), ("sl_ramp", ctypes.c_int32),
+        ("damp_hard", ctypes.c_int32), ("damp_mean", ctypes.c_int32), ("damp_follow", ctypes.c_int32),
     ]
 
 
@@ -77,6 +82,11 @@ class Deck:
     s2_period: float = 0.0
     s2_start: float = 0.0
     sl_amplitude: float = 0.0
+    sl_period: float = 1.0
+    sl_delta: float = 0.02
+    sl_start: float = 0.0
+    sl_bodyforce: bool = True
+    sl_ramp: bool = False
     # &time_control (:31-36)
     deltat: float = 0.0
     cflmax: float = 1.0
@@ -98,11 +108,16 @@ class Deck:
     kpeak: float = 4.0
     tg_amplitude: float = 1.0
     tg_mode: int = 1
-    # &damping (:39-40)
+    # &damping (:40-41)
     damp_kpar: float = 0.0
     damp_kperp: float = 0.0
     damp_rate: float = 0.0
     damp_hard: bool = False
+    damp_edge: float = 0.0
+    damp_norm: float = 2.0
+    damp_mean: bool = False
+    damp_follow: bool = True
+    damp_angle: float = 0.0
 
     # ---------------------------------------------------------------- derived
     @property
@@ -153,8 +168,12 @@ class Deck:
         """The deck checks of read_input (src/hst_input.f90:59-90) that concern this path."""
         if self.sl_amplitude != 0.0 and (self.s2_amplitude != 0.0 or self.s2_mean != 0.0):
             raise ValueError("the Stokes layer and the spanwise shear S2 cannot both be active")
+        if self.sl_amplitude != 0.0 and self.sl_period <= 0.0:
+            raise ValueError("sl_period must be positive")
         if (self.damp_kpar > 0.0) != (self.damp_kperp > 0.0):
             raise ValueError("damp_kpar and damp_kperp must both be positive (damping on) or both zero (off)")
+        if self.damp_norm < 1.0 or self.damp_edge < 0.0 or self.damp_edge >= 1.0 or self.damp_rate < 0.0:
+            raise ValueError("damping needs damp_norm >= 1, 0 <= damp_edge < 1, damp_rate >= 0")
         if self.initial_condition.strip() not in ("random", "taylor_green"):
             raise ValueError("initial_condition must be 'random' or 'taylor_green'")
         if self.tg_mode < 1:
@@ -283,11 +302,15 @@ def write_deck(d: Deck, path: str, comment: str = "") -> None:
 
     groups = {
         "mesh": ["nx", "ny", "nz", "alfa0", "beta0", "ly", "ystretch"],
-        "physics": ["re", "S", "linear", "exact_shift", "s2_mean", "s2_amplitude", "s2_period", "s2_start"],
+        "physics": ["re", "S", "linear", "exact_shift", "s2_mean", "s2_amplitude", "s2_period", "s2_start",
+                    "sl_amplitude", "sl_period", "sl_delta", "sl_start", "sl_bodyforce", "sl_ramp"],
         "time_control": ["deltat", "cflmax", "t_max", "nstep", "dt_stat", "dt_field", "dt_save", "time",
                          "time_from_restart", "restart_from_file"],
         "init": ["initial_condition", "amplitude", "seed", "kpeak"],
     }
+    if d.damp_kpar > 0.0:
+        groups["damping"] = ["damp_kpar", "damp_kperp", "damp_rate", "damp_hard", "damp_edge", "damp_norm",
+                             "damp_mean", "damp_follow", "damp_angle"]
     with open(path, "w") as fh:
         for line in comment.splitlines():
             fh.write(f"! {line}\n")

@@ -20,6 +20,7 @@ program hst
   use hst_stats
   use hst_pressure
   use hst_timer
+  use hst_stokes, only: stokes_apply
 
   implicit none
 
@@ -45,6 +46,8 @@ program hst
     if (has_terminal) print *, '   restart_from_file = .false.: generating initial field'
     call generate_initial_field()
   end if
+  ! the Stokes layer's profile imposed on the loaded field (stokes_setup, stokes_apply)
+  call stokes_apply()
   call fill_ghosts(1)
   call fill_ghosts(2)
   call fill_ghosts(3)

@@ -18,6 +18,10 @@ module hst_gpu_api
     real(c_double) :: amplitude, kpeak, tg_amplitude
     real(c_double) :: sl_amplitude
     real(c_double) :: damp_kpar, damp_kperp
+    real(c_double) :: sl_period, sl_delta, sl_start
+    real(c_double) :: damp_rate, damp_edge, damp_norm, damp_angle
+    integer(c_int32_t) :: sl_bodyforce, sl_ramp
+    integer(c_int32_t) :: damp_hard, damp_mean, damp_follow
   end type hst_deck_t
 
   type, bind(c) :: hst_grid_info_t
@@ -134,6 +138,17 @@ module hst_gpu_api
       import :: c_int, c_double
       real(c_double), value :: time
       real(c_double), intent(out) :: out(10)
+    end function
+    integer(c_int) function hst_gpu_stokes_apply(time) bind(c, name='hst_gpu_stokes_apply')
+      import :: c_int, c_double
+      real(c_double), value :: time
+    end function
+    integer(c_int) function hst_gpu_damping_mask() bind(c, name='hst_gpu_damping_mask')
+      import :: c_int
+    end function
+    integer(c_int) function hst_gpu_outstats_extra(out) bind(c, name='hst_gpu_outstats_extra')
+      import :: c_int, c_double
+      real(c_double), intent(out) :: out(8)
     end function
     integer(c_int) function hst_gpu_pressure(time, dst) bind(c, name='hst_gpu_pressure')
       import :: c_int, c_double

@@ -68,12 +68,20 @@ contains
       print *, "ERROR: restart_remap must be 'strict' (a mode-index remap is not part of this build)"
       error stop 1
     end if
-    if (sl_amplitude /= 0.0d0) then
-      print *, 'ERROR: the Stokes layer (sl_amplitude) is not part of this build'
+    if (sl_amplitude /= 0.0d0 .and. (s2_amplitude /= 0.0d0 .or. s2_mean /= 0.0d0)) then
+      print *, 'ERROR: the Stokes layer and the spanwise shear S2 cannot both be active'
       error stop 1
     end if
-    if (damp_kpar > 0.0d0 .or. damp_kperp > 0.0d0) then
-      print *, 'ERROR: large-scale damping is not part of this build'
+    if (sl_amplitude /= 0.0d0 .and. sl_period <= 0.0d0) then
+      print *, 'ERROR: sl_period must be positive'
+      error stop 1
+    end if
+    if ((damp_kpar > 0.0d0) .neqv. (damp_kperp > 0.0d0)) then
+      print *, 'ERROR: damp_kpar and damp_kperp must both be positive (damping on) or both zero (off)'
+      error stop 1
+    end if
+    if (damp_norm < 1.0d0 .or. damp_edge < 0.0d0 .or. damp_edge >= 1.0d0 .or. damp_rate < 0.0d0) then
+      print *, 'ERROR: damping needs damp_norm >= 1, 0 <= damp_edge < 1, damp_rate >= 0'
       error stop 1
     end if
     if (trim(initial_condition) /= 'random' .and. trim(initial_condition) /= 'taylor_green') then
@@ -134,6 +142,12 @@ contains
       '   exact_shift =', exact_shift
     if (s2_amplitude /= 0.0d0 .or. s2_mean /= 0.0d0) write (*, '(A,F10.6,A,F10.6,A,F10.6,A,F10.6)') &
       '   S2: mean=', s2_mean, '   amplitude=', s2_amplitude, '   period=', s2_period, '   start=', s2_start
+    if (sl_amplitude /= 0.0d0) write (*, '(A,F8.4,A,F8.4,A,F8.4,A,F8.4,A,L1,A,L1)') &
+      '   Stokes layer: amplitude=', sl_amplitude, '  period=', sl_period, '  delta=', sl_delta, &
+      '  start=', sl_start, '  bodyforce=', sl_bodyforce, '  ramp=', sl_ramp
+    if (damp_kpar > 0.0d0) write (*, '(A,F8.4,A,F8.4,A,F6.2,A,L1,A,F8.3,A,F6.3,A,L1,A,L1,A,F8.3)') &
+      '   damping: k_par <', damp_kpar, '  k_perp <', damp_kperp, '  norm=', damp_norm, '  hard=', damp_hard, &
+      '  rate=', damp_rate, '  edge=', damp_edge, '  mean=', damp_mean, '  follow=', damp_follow, '  angle=', damp_angle
     write (*, '(A,F10.6,A,F10.6,A,F10.3)') '   deltat=', deltat, '   cflmax=', cflmax, '   t_max =', t_max
     write (*, '(A,F10.4,A,F10.4,A,F10.4)') '   dt_stat=', dt_stat, '  dt_field=', dt_field, '  dt_save=', dt_save
     write (*, '(A,I10,A,F10.0,A,L1)') '   nstep =', nstep, '   wall_max =', wall_max, '   restart_from_file = ', &

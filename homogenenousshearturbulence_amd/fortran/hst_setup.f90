@@ -28,6 +28,11 @@ contains
     d%s2_start = s2_start
     d%amplitude = amplitude; d%kpeak = kpeak; d%tg_amplitude = tg_amplitude
     d%sl_amplitude = sl_amplitude; d%damp_kpar = damp_kpar; d%damp_kperp = damp_kperp
+    d%sl_period = sl_period; d%sl_delta = sl_delta; d%sl_start = sl_start
+    d%sl_bodyforce = merge(1, 0, sl_bodyforce); d%sl_ramp = merge(1, 0, sl_ramp)
+    d%damp_rate = damp_rate; d%damp_edge = damp_edge; d%damp_norm = damp_norm; d%damp_angle = damp_angle
+    d%damp_hard = merge(1, 0, damp_hard); d%damp_mean = merge(1, 0, damp_mean)
+    d%damp_follow = merge(1, 0, damp_follow)
   end function deck_struct
 
   ! The device state (fields, transform and solver buffers, tables), zeroed,

@@ -34,6 +34,7 @@ EXPORTS = (
     "hst_gpu_buildrhs_prepare", "hst_gpu_buildrhs", "hst_gpu_shear_shift", "hst_gpu_line_solve", "hst_gpu_linsolve",
     "hst_gpu_timestep", "hst_gpu_outstats", "hst_gpu_set_chunk_rows", "hst_gpu_memory", "hst_gpu_profile",
     "hst_gpu_profile_count", "hst_gpu_profile_entry", "hst_gpu_pressure", "hst_gpu_spectrum",
+    "hst_gpu_stokes_apply", "hst_gpu_damping_mask", "hst_gpu_outstats_extra",
 )
 
 
@@ -84,6 +85,9 @@ def lib() -> ctypes.CDLL:
         "hst_gpu_profile_entry": [i, ctypes.c_char_p, i, ctypes.POINTER(ctypes.c_int64), dp, dp],
         "hst_gpu_pressure": [d, i],
         "hst_gpu_spectrum": [vp, ctypes.POINTER(ctypes.c_int32)],
+        "hst_gpu_stokes_apply": [d],
+        "hst_gpu_damping_mask": [],
+        "hst_gpu_outstats_extra": [dp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -293,6 +297,21 @@ class Solver:
         _check(lib().hst_gpu_spectrum(E.ctypes.data, ctypes.byref(n)), "hst_gpu_spectrum")
         return E, n.value
 
+    def stokes_apply(self) -> None:
+        """stokes_apply at self.time (src/hst_stokes.f90:103-118), ghosts included."""
+        _check(lib().hst_gpu_stokes_apply(self.time), "hst_gpu_stokes_apply")
+
+    def damping_mask(self) -> None:
+        _check(lib().hst_gpu_damping_mask(), "hst_gpu_damping_mask")
+
+    def outstats_extra(self) -> np.ndarray:
+        """The Stokes-layer region line and the damping line of the last
+        outstats: energy out/in, dissipation out/in, direction (deg), damped
+        columns, half their energy, half their lost power."""
+        out = np.zeros(8)
+        _check(lib().hst_gpu_outstats_extra(_dp(out)), "hst_gpu_outstats_extra")
+        return out
+
     def sync(self) -> None:
         _check(lib().hst_gpu_sync(), "hst_gpu_sync")
 
@@ -322,6 +341,7 @@ def run_deck(deck: Deck, precision: int = FP64, nsteps: int | None = None, solve
     s = solver or Solver(deck, precision)
     s.time = deck.time
     s.initial_field()
+    s.stokes_apply()  # the layer profile imposed on the fresh field (src/hst.f90:76-77)
     s.fill_all_ghosts()
     dt = deck.deltat
     istep = 0

@@ -32,9 +32,10 @@ extern "C" {
 #endif
 
 /* The deck values the substep needs: the namelist groups &mesh, &physics,
- * &init of hst.in (src/hst_input.f90:19-23, defaults :26-42).  Logicals are
- * 0/1.  The hooks that are off in every deck this library serves (Stokes
- * layer, large-scale damping) must be zero; hst_gpu_init rejects them. */
+ * &init and &damping of hst.in (src/hst_input.f90:19-24, defaults :26-42,
+ * checks :62-82).  Logicals are 0/1.  The Stokes layer (src/hst_stokes.f90)
+ * is on when sl_amplitude != 0; the large-scale damping (src/hst_damping.f90)
+ * when damp_kpar, damp_kperp > 0 and (damp_hard or damp_rate > 0). */
 typedef struct hst_deck {
   int32_t nx, ny, nz;              /* &mesh: modes in x, points in y, modes in z */
   int32_t linear, exact_shift;     /* &physics logicals */
@@ -43,8 +44,12 @@ typedef struct hst_deck {
   double alfa0, beta0, ly, ystretch;
   double re, S, s2_mean, s2_amplitude, s2_period, s2_start;
   double amplitude, kpeak, tg_amplitude;
-  double sl_amplitude;             /* must be 0 (Stokes layer, src/hst_stokes.f90) */
-  double damp_kpar, damp_kperp;    /* must be 0 (damping, src/hst_damping.f90) */
+  double sl_amplitude;             /* Stokes layer amplitude A (0: off) */
+  double damp_kpar, damp_kperp;    /* damping region semi-axes (both 0: off) */
+  double sl_period, sl_delta, sl_start;                /* layer period T (> 0), width, start t0 */
+  double damp_rate, damp_edge, damp_norm, damp_angle;  /* soft rate, edge width, norm p >= 1, angle (deg) */
+  int32_t sl_bodyforce, sl_ramp;                       /* &physics logicals of the layer */
+  int32_t damp_hard, damp_mean, damp_follow;           /* &damping logicals */
 } hst_deck;
 
 /* Derived sizes and grid tables (src/hst_input.f90:92-115, src/hst_setup.f90:42-64). */
@@ -151,6 +156,28 @@ int hst_gpu_timestep(double deltat, double *time, double *cfl);
  * vw, uu, vv, ww, uw (box means, as written to Runtimedata and
  * variances_runtime.dat), mfx, mfz. */
 int hst_gpu_outstats(double time, double out[10]);
+
+/* stokes_apply (src/hst_stokes.f90:103-118) at `time`: the (0,0) mode of w
+ * set to the layer profile W(y, time) on every row (without the body force or
+ * within two periods of sl_start) or on the rows 0 and ny-1, its ghost rows
+ * refilled.  hst_gpu_timestep calls it after each substep's solve; a host
+ * calls it once after loading a field (src/hst.f90:76-77).  No-op without
+ * the layer. */
+int hst_gpu_stokes_apply(double time);
+
+/* damping_mask (src/hst_damping.f90:119-133): the right-hand sides and
+ * carried terms of the hard-masked columns set to zero (for a host that runs
+ * the pieces of a substep; hst_gpu_timestep does it inside buildrhs).  The
+ * mask is the one of the last hst_gpu_buildrhs_prepare. */
+int hst_gpu_damping_mask(void);
+
+/* The extra lines of the last hst_gpu_outstats (src/hst_stats.f90:270-284):
+ * out[0..3] = energy outside and inside |y - ly/2| < 8 delta, dissipation
+ * outside and inside (statistics/stokes_runtime.dat; zero without the layer);
+ * out[4..7] = the damping direction in degrees, the number of damped columns,
+ * half their energy and half the power they lose (damping_runtime.dat; zero
+ * without damping). */
+int hst_gpu_outstats_extra(double out[8]);
 
 /* The fluctuation pressure of the current velocity (ghost rows filled) at
  * `time` into field dst (compute_pressure, src/hst_pressure.f90:35-112):

@@ -117,11 +117,19 @@ typedef struct hsto {
   cplx *prod;  /* 6 product spectra, reference layout */
   fftplan pz, px;
   double cfl;
+  /* physics hooks (src/hst_stokes.f90, src/hst_damping.f90) */
+  double *gdamp;    /* gdamp(-nz:nz, 0:nx) */
+  double *fy;       /* fy(-2:ny+1) -> fy[iy+2] */
+  int *inlayer;     /* inlayer(0:ny-1) */
+  double theta_last;
+  int theta_set;
+  double extra[8];  /* stokes_runtime.dat, damping_runtime.dat values of the last outstats */
 } hsto;
 
 #define FIDX(o, ix, iz, iy) ((((size_t)(ix)) * (o)->NZm + (size_t)((iz) + (o)->nz)) * (o)->NYg + (size_t)((iy) + 2))
 #define DER(o, iy, n, j) ((o)->der[(size_t)(iy) + (size_t)(o)->ny * ((size_t)(n) + 4 * (size_t)((j) + 2))])
 #define K2(o, iz, ix) ((o)->k2[(size_t)((iz) + (o)->nz) + (size_t)(o)->NZm * (size_t)(ix)])
+#define GD(o, iz, ix) ((o)->gdamp[(size_t)((iz) + (o)->nz) + (size_t)(o)->NZm * (size_t)(ix)])
 
 /* RK_rai (src/hst_params.f90:108-111), column i = substep. */
 static const double RK_rai[3][3] = {
@@ -311,6 +319,12 @@ hsto *hsto_create(const hst_deck *d) {
   o->prod = (cplx *)calloc(6 * nf, sizeof(cplx));
   plan_init(&o->pz, o->nzd);
   plan_init(&o->px, 2 * o->nxd);
+  o->gdamp = (double *)calloc((size_t)o->NZm * o->NXm, sizeof(double));
+  o->fy = (double *)calloc(o->NYg, sizeof(double));
+  o->inlayer = (int *)calloc(o->ny, sizeof(int));
+  /* stokes_setup (src/hst_stokes.f90:76-86) */
+  if (d->sl_amplitude != 0.0)
+    for (int iy = 0; iy < o->ny; iy++) o->inlayer[iy] = fabs(o->y[iy + 2] - 0.5 * d->ly) < 8.0 * d->sl_delta;
   return o;
 }
 
@@ -319,6 +333,7 @@ void hsto_free(hsto *o) {
   free(o->y); free(o->dyl); free(o->der); free(o->k2);
   for (int i = 1; i <= 8; i++) free(o->f[i]);
   free(o->prod);
+  free(o->gdamp); free(o->fy); free(o->inlayer);
   plan_free(&o->pz); plan_free(&o->px);
   free(o);
 }
@@ -721,15 +736,125 @@ double hsto_nonlinear(hsto *o, int mode, int want_cfl) {
 
 /* ------------------------------------------------------------------ equations */
 
-/* buildrhs_prepare (src/hst_equations.f90:92-141), no damping, no Stokes force */
+/* ------------------------------------------------------------------ physics hooks */
+
+/* stokes_active, damping_active, damping_soft_factor, damping_theta
+ * (src/hst_stokes.f90:48-50, src/hst_damping.f90:45-67) */
+static int stokes_active(const hsto *o) { return o->d.sl_amplitude != 0.0; }
+static int damping_active(const hsto *o) {
+  return o->d.damp_kpar > 0.0 && o->d.damp_kperp > 0.0 && (o->d.damp_hard || o->d.damp_rate > 0.0);
+}
+static double damping_soft_factor(const hsto *o) { return (damping_active(o) && !o->d.damp_hard) ? 1.0 : 0.0; }
+static double damping_theta(const hsto *o, double t) {
+  if (o->d.damp_follow) return atan2(s2_of(o, t), o->d.S);
+  return o->d.damp_angle * PI / 180.0;
+}
+
+/* ramp, stokes_profile, stokes_force_at (src/hst_stokes.f90:52-88) */
+static double sl_ramp(const hsto *o, double t) {
+  const hst_deck *d = &o->d;
+  if (t < d->sl_start) return 0.0;
+  if (d->sl_ramp) {
+    double tau = fmin((t - d->sl_start) / d->sl_period, 1.0);
+    return 3.0 * tau * tau - 2.0 * tau * tau * tau;
+  }
+  return 1.0;
+}
+static double stokes_profile(const hsto *o, double yy, double t) {
+  const hst_deck *d = &o->d;
+  const double omega = TWOPI / d->sl_period, e = (yy - 0.5 * d->ly) / d->sl_delta;
+  const double s = sqrt(e * e + 0.1 * 0.1);
+  return sl_ramp(o, t) * d->sl_amplitude * exp(-s) * cos(omega * (t - d->sl_start) - s);
+}
+static double stokes_force_at(const hsto *o, double yy, double t) {
+  const hst_deck *d = &o->d;
+  const double omega = TWOPI / d->sl_period, eta = yy - 0.5 * d->ly, e = eta / d->sl_delta;
+  const double s = sqrt(e * e + 0.1 * 0.1);
+  const double s1 = (eta / (d->sl_delta * d->sl_delta)) / s;
+  const double s2 = ((0.1 * 0.1) / (d->sl_delta * d->sl_delta)) / (s * s * s);
+  const double theta = omega * (t - d->sl_start) - s;
+  return sl_ramp(o, t) * d->sl_amplitude * exp(-s) *
+         ((2.0 * o->ni * s1 * s1 - omega) * sin(theta) - o->ni * s2 * (sin(theta) - cos(theta)));
+}
+
+/* stokes_force (src/hst_stokes.f90:90-101): fy with periodic ghost rows */
+static void stokes_force(hsto *o, double t) {
+  if (!(stokes_active(o) && o->d.sl_bodyforce)) return;
+  const int ny = o->ny;
+  for (int iy = 0; iy < ny; iy++) o->fy[iy + 2] = stokes_force_at(o, o->y[iy + 2], t);
+  o->fy[0] = o->fy[ny - 2 + 2]; o->fy[1] = o->fy[ny - 1 + 2];
+  o->fy[ny + 2] = o->fy[2]; o->fy[ny + 3] = o->fy[3];
+}
+
+/* stokes_apply (src/hst_stokes.f90:103-118) at t, then fill_ghosts(3) */
+void hsto_stokes_apply(hsto *o, double t) {
+  if (!stokes_active(o)) return;
+  const int whole = (!o->d.sl_bodyforce) || (t - o->d.sl_start <= 2.0 * o->d.sl_period);
+  cplx *W = o->f[3];
+  for (int iy = 0; iy < o->ny; iy++)
+    if (whole || iy == 0 || iy == o->ny - 1) W[FIDX(o, 0, 0, iy)] = stokes_profile(o, o->y[iy + 2], t);
+  fill_ghosts_t(o, 3, t);
+}
+
+/* damping_update (src/hst_damping.f90:78-115) */
+static void damping_update(hsto *o, double t) {
+  if (!damping_active(o)) return;
+  const hst_deck *d = &o->d;
+  const double theta = damping_theta(o, t);
+  if (o->theta_set && theta == o->theta_last) return;
+  o->theta_set = 1;
+  o->theta_last = theta;
+  const double c = cos(theta), sn = sin(theta);
+  for (int ix = 0; ix <= o->nx; ix++) {
+    const double kx = d->alfa0 * ix;
+    for (int iz = -o->nz; iz <= o->nz; iz++) {
+      const double kz = d->beta0 * iz;
+      const double kpar = kx * c + kz * sn, kperp = -kx * sn + kz * c;
+      const double r = pow(pow(fabs(kpar) / d->damp_kpar, d->damp_norm) + pow(fabs(kperp) / d->damp_kperp, d->damp_norm),
+                           1.0 / d->damp_norm);
+      double g = 0.0;
+      if (r < 1.0 - 1.0e-6) {
+        if (d->damp_hard) {
+          g = 1.0;
+        } else if (r <= 1.0 - d->damp_edge) {
+          g = d->damp_rate;
+        } else {
+          const double x = (1.0 - r) / d->damp_edge;
+          g = d->damp_rate * x * x * (3.0 - 2.0 * x);
+        }
+      }
+      if (ix == 0 && iz == 0 && !d->damp_mean) g = 0.0;
+      GD(o, iz, ix) = g;
+    }
+  }
+}
+
+/* damping_mask (src/hst_damping.f90:119-133) */
+void hsto_damping_mask(hsto *o) {
+  if (!(damping_active(o) && o->d.damp_hard)) return;
+  cplx *R1 = o->f[HST_RHS1], *R2 = o->f[HST_RHS2], *O1 = o->f[HST_OLD1], *O2 = o->f[HST_OLD2];
+  for (int ix = 0; ix <= o->nx; ix++)
+    for (int iz = -o->nz; iz <= o->nz; iz++) {
+      if (!(GD(o, iz, ix) > 0.0)) continue;
+      for (int iy = 0; iy < o->ny; iy++) {
+        size_t c = FIDX(o, ix, iz, iy);
+        R1[c] = 0.0; R2[c] = 0.0; O1[c] = 0.0; O2[c] = 0.0;
+      }
+    }
+}
+
+/* buildrhs_prepare (src/hst_equations.f90:92-141) at `time`, after the
+ * substep's damping_update (:64) */
 void hsto_buildrhs_prepare(hsto *o, const double ODE[3], double deltat, double time) {
-  const double s2now = s2_of(o, time), ni = o->ni, S = o->d.S, g = 0.0;
+  damping_update(o, time);
+  stokes_force(o, time);
+  const double s2now = s2_of(o, time), ni = o->ni, S = o->d.S, gsoft = damping_soft_factor(o);
   cplx *V1 = o->f[1], *V2 = o->f[2], *V3 = o->f[3];
   cplx *R1 = o->f[HST_RHS1], *R2 = o->f[HST_RHS2], *O1 = o->f[HST_OLD1], *O2 = o->f[HST_OLD2];
 #pragma omp parallel for collapse(2)
   for (int ix = 0; ix <= o->nx; ix++)
     for (int iz = -o->nz; iz <= o->nz; iz++) {
-      const double kk = K2(o, iz, ix);
+      const double kk = K2(o, iz, ix), g = gsoft * GD(o, iz, ix);
       const cplx ialfa = I * (ix * o->d.alfa0), ibeta = I * (iz * o->d.beta0);
       for (int iy = 0; iy < o->ny; iy++) {
         size_t c = FIDX(o, ix, iz, iy);
@@ -750,7 +875,7 @@ void hsto_buildrhs_prepare(hsto *o, const double ODE[3], double deltat, double t
             f = creal(V1[c + j]) + I * creal(V3[c + j]);
             unkn = unkn + DER(o, iy, 0, j) * f;
             impl = impl + ni * DER(o, iy, 2, j) * f;
-            force = force + DER(o, iy, 0, j) * 0.0;
+            force = force + DER(o, iy, 0, j) * o->fy[iy + j + 2];
           } else {
             f = ibeta * V1[c + j] - ialfa * V3[c + j];
             unkn = unkn + DER(o, iy, 0, j) * f;
@@ -769,6 +894,8 @@ void hsto_buildrhs_prepare(hsto *o, const double ODE[3], double deltat, double t
 /* buildrhs(ODE, g) (src/hst_equations.f90:151-213), no Stokes layer */
 void hsto_buildrhs(hsto *o, const double ODE[3], int g) {
   cplx *R1 = o->f[HST_RHS1], *R2 = o->f[HST_RHS2], *O1 = o->f[HST_OLD1], *O2 = o->f[HST_OLD2];
+  /* with the layer's body force the mean w equation drops the Reynolds-stress divergence */
+  const int no_mean_vw = stokes_active(o) && o->d.sl_bodyforce;
   const size_t nf = (size_t)o->NXm * o->NZm * o->NYg;
 #pragma omp parallel for collapse(2)
   for (int ix = 0; ix <= o->nx; ix++)
@@ -792,7 +919,11 @@ void hsto_buildrhs(hsto *o, const double ODE[3], int g) {
             case 2: rhsu = 0.0; rhsw = 0.0; expl = kk * d1; break;
             case 3: rhsu = 0.0; rhsw = -(ibeta * d0); expl = ibeta * ibeta * d1; break;
             case 4: rhsu = -d1; rhsw = 0.0; expl = ialfa * d2 + ialfa * kk * d0; break;
-            case 5: rhsu = 0.0; rhsw = -d1; expl = ibeta * d2 + ibeta * kk * d0; break;
+            case 5:
+              rhsu = 0.0; rhsw = -d1;
+              if (no_mean_vw && ix == 0 && iz == 0) rhsw = 0.0;
+              expl = ibeta * d2 + ibeta * kk * d0;
+              break;
             default: rhsu = -(ibeta * d0); rhsw = -(ialfa * d0); expl = 2.0 * ialfa * ibeta * d1; break;
           }
           r2 = r2 + ODE[1] * expl;
@@ -903,10 +1034,12 @@ void hsto_timestep(hsto *o, double deltat, double *time, double *cfl) {
       hsto_buildrhs(o, ODE, 1);
       hsto_buildrhs(o, ODE, 2);
     }
+    hsto_damping_mask(o);
     double dt_sub = 2.0 / ODE[0] * deltat;
     hsto_shear_shift(o, dt_sub, *time);
     *time = *time + dt_sub;
     hsto_linsolve(o, ODE[0] / deltat, *time);
+    hsto_stokes_apply(o, *time);
   }
 }
 
@@ -919,64 +1052,103 @@ void hsto_fill_all_ghosts(hsto *o, double t) {
 
 /* The first reduction of outstats (src/hst_stats.f90:153-198) without Stokes
  * layer and damping: raw sums eps(spectral), uv, uu, vv, ww, vw, uw, mfx, mfz. */
-static void outstats_raw(hsto *o, double out[10]) {
+/* The first reduction of outstats (src/hst_stats.f90:145-198):
+ * out = eps uv uu vv ww vw uw mfx mfz q_in q_out e_in e_out ed pd ncol_damped */
+static void outstats_raw(hsto *o, int inc00, double out[16]) {
   double eps = 0, uv = 0, uu = 0, vv = 0, ww = 0, vw = 0, uw = 0, mfx = 0, mfz = 0;
+  double q_in = 0, q_out = 0, e_in = 0, e_out = 0, ed = 0, pd = 0, nd = 0;
+  const double gsoft = damping_soft_factor(o);
   cplx *V1 = o->f[1], *V2 = o->f[2], *V3 = o->f[3];
   for (int ix = 0; ix <= o->nx; ix++)
-    for (int iz = -o->nz; iz <= o->nz; iz++)
+    for (int iz = -o->nz; iz <= o->nz; iz++) {
+      const double gd = GD(o, iz, ix);
+      if (gd > 0.0) nd += 1.0;
       for (int iy = 0; iy < o->ny; iy++) {
         size_t c = FIDX(o, ix, iz, iy);
         cplx cu = V1[c], cv = V2[c], cw = V3[c];
         if (ix == 0 && iz == 0) {
           mfx += creal(cu) * o->dyl[iy];
           mfz += creal(cw) * o->dyl[iy];
-          uu += o->dyl[iy] * creal(cu * conj(cu));
-          ww += o->dyl[iy] * creal(cw * conj(cw));
-          uw += o->dyl[iy] * creal(cu * conj(cw));
+          if (gd > 0.0) {
+            double qc = o->dyl[iy] * creal(cu * conj(cu) + cw * conj(cw));
+            ed += qc;
+            pd += 2.0 * gsoft * gd * qc;
+          }
+          if (inc00) {
+            uu += o->dyl[iy] * creal(cu * conj(cu));
+            ww += o->dyl[iy] * creal(cw * conj(cw));
+            uw += o->dyl[iy] * creal(cu * conj(cw));
+          }
           continue;
         }
         double w = 2.0 * o->dyl[iy];
         if (ix == 0) w = o->dyl[iy];
+        const double q = w * creal(cu * conj(cu) + cv * conj(cv) + cw * conj(cw));
+        if (gd > 0.0) {
+          ed += q;
+          pd += 2.0 * gsoft * gd * q;
+        }
         uu += w * creal(cu * conj(cu));
         vv += w * creal(cv * conj(cv));
         ww += w * creal(cw * conj(cw));
         uv += w * creal(cu * conj(cv));
         vw += w * creal(cw * conj(cv));
         uw += w * creal(cu * conj(cw));
-        eps += w * K2(o, iz, ix) * creal(cu * conj(cu) + cv * conj(cv) + cw * conj(cw));
+        const double e = w * K2(o, iz, ix) * creal(cu * conj(cu) + cv * conj(cv) + cw * conj(cw));
+        eps += e;
+        if (o->inlayer[iy] == 1) {
+          q_in += q;
+          e_in += e;
+        } else {
+          q_out += q;
+          e_out += e;
+        }
       }
+    }
   out[0] = eps; out[1] = uv; out[2] = uu; out[3] = vv; out[4] = ww; out[5] = vw; out[6] = uw; out[7] = mfx; out[8] = mfz;
+  out[9] = q_in; out[10] = q_out; out[11] = e_in; out[12] = e_out; out[13] = ed; out[14] = pd; out[15] = nd;
 }
 
-/* outstats (src/hst_stats.f90:136-270) at `time` (the wrap of the DY solves):
+/* outstats (src/hst_stats.f90:136-285) at `time` (the wrap of the DY solves):
  * out = energy, diss, uv, vw, uu, vv, ww, uw, mfx, mfz as written to
- * Runtimedata / variances_runtime.dat. */
+ * Runtimedata / variances_runtime.dat; the region and damping lines go to
+ * o->extra (hsto_outstats_extra). */
 void hsto_outstats_at(hsto *o, double time, double out[10]) {
-  double raw[10];
-  outstats_raw(o, raw);
+  if (!o->theta_set) damping_update(o, time); /* damping_setup's table at the start */
+  const int inc00 = !stokes_active(o);
+  double raw[16];
+  outstats_raw(o, inc00, raw);
   double eps = raw[0], uv = raw[1], uu = raw[2], vv = raw[3], ww = raw[4], vw = raw[5], uw = raw[6];
-  double mfx = raw[7], mfz = raw[8];
+  double mfx = raw[7], mfz = raw[8], e_in = raw[11], e_out = raw[12];
   double sx, sz;
   hsto_shear_shifts(o, time, &sx, &sz);
   for (int c = 1; c <= 3; c++) {
     hsto_line_solve(o, HST_KIND_DY, 0.0, c, HST_SCRATCH, sx, sz);
     const cplx *R = o->f[HST_SCRATCH];
-    double grad = 0.0;
+    double grad = 0.0, g_in = 0.0, g_out = 0.0;
     for (int ix = 0; ix <= o->nx; ix++)
       for (int iz = -o->nz; iz <= o->nz; iz++)
         for (int iy = 0; iy < o->ny; iy++) {
+          if (ix == 0 && iz == 0 && !inc00) continue;
           double w = 2.0 * o->dyl[iy];
           if (ix == 0) w = o->dyl[iy];
           cplx dq = R[FIDX(o, ix, iz, iy)];
-          grad += w * creal(dq * conj(dq));
+          const double q = w * creal(dq * conj(dq));
+          grad += q;
+          if (o->inlayer[iy] == 1) g_in += q;
+          else g_out += q;
         }
     eps += grad;
+    e_in += g_in;
+    e_out += g_out;
   }
-  double ly_w = 0.0;
-  for (int iy = 0; iy < o->ny; iy++) ly_w += o->dyl[iy];
-  uu = uu - mfx * mfx / ly_w;
-  ww = ww - mfz * mfz / ly_w;
-  uw = uw - mfx * mfz / ly_w;
+  if (inc00) {
+    double ly_w = 0.0;
+    for (int iy = 0; iy < o->ny; iy++) ly_w += o->dyl[iy];
+    uu = uu - mfx * mfx / ly_w;
+    ww = ww - mfz * mfz / ly_w;
+    uw = uw - mfx * mfz / ly_w;
+  }
   double q2 = uu + vv + ww, inv_ly = 1.0 / o->d.ly;
   out[0] = 0.5 * q2 * inv_ly;
   out[1] = o->ni * eps * inv_ly;
@@ -988,7 +1160,30 @@ void hsto_outstats_at(hsto *o, double time, double out[10]) {
   out[7] = uw * inv_ly;
   out[8] = mfx;
   out[9] = mfz;
+  for (int k = 0; k < 8; k++) o->extra[k] = 0.0;
+  if (stokes_active(o)) {
+    double l_in = 0.0, l_out = 0.0;
+    for (int iy = 0; iy < o->ny; iy++) {
+      if (o->inlayer[iy] == 1) l_in += o->dyl[iy];
+      else l_out += o->dyl[iy];
+    }
+    o->extra[0] = 0.5 * raw[10] / l_out;
+    o->extra[1] = 0.5 * raw[9] / l_in;
+    o->extra[2] = o->ni * e_out / l_out;
+    o->extra[3] = o->ni * e_in / l_in;
+  }
+  if (damping_active(o)) {
+    o->extra[4] = damping_theta(o, time) * 180.0 / 3.141592653589793;
+    o->extra[5] = raw[15];
+    o->extra[6] = 0.5 * raw[13];
+    o->extra[7] = 0.5 * raw[14];
+  }
 }
+
+void hsto_outstats_extra(const hsto *o, double out[8]) { memcpy(out, o->extra, sizeof(double) * 8); }
+
+/* gdamp(-nz:nz, 0:nx) as last built (zero before the first damping_update) */
+double *hsto_gdamp(hsto *o) { return o->gdamp; }
 
 /* compute_pressure (src/hst_pressure.f90:35-112) into field dst at `time`
  * (the velocity's ghost rows filled): the products once more, the pressure

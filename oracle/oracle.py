@@ -63,6 +63,11 @@ def lib():
         L.hsto_set_threads.argtypes = [i]
         L.hsto_max_threads.restype = i
         L.hsto_pressure.argtypes = [vp, d, i]
+        L.hsto_stokes_apply.argtypes = [vp, d]
+        L.hsto_damping_mask.argtypes = [vp]
+        L.hsto_outstats_extra.argtypes = [vp, vp]
+        L.hsto_gdamp.restype = vp
+        L.hsto_gdamp.argtypes = [vp]
         L.hsto_column_energy.argtypes = [vp, vp]
         # learned closure
         L.hsto_filter_weights.restype = i
@@ -166,6 +171,23 @@ class Oracle:
         lib().hsto_pressure(self.h, self.time, dst)
         return self.field(dst).copy()
 
+    def stokes_apply(self) -> None:
+        lib().hsto_stokes_apply(self.h, self.time)
+
+    def damping_mask(self) -> None:
+        lib().hsto_damping_mask(self.h)
+
+    def outstats_extra(self) -> np.ndarray:
+        out = np.zeros(8)
+        lib().hsto_outstats_extra(self.h, out.ctypes.data)
+        return out
+
+    def gdamp(self) -> np.ndarray:
+        """gdamp[ix, iz + nz] as last built."""
+        n = (self.deck.nx + 1) * (2 * self.deck.nz + 1)
+        buf = (ctypes.c_double * n).from_address(lib().hsto_gdamp(self.h))
+        return np.frombuffer(buf, dtype=np.float64).reshape((self.deck.nx + 1, 2 * self.deck.nz + 1)).copy()
+
     def column_energy(self) -> np.ndarray:
         """E[ix, iz + nz] of the current velocity (one outstats sample)."""
         E = np.zeros((self.deck.nx + 1, 2 * self.deck.nz + 1))
@@ -196,6 +218,7 @@ def run_deck(deck: Deck, nsteps: int | None = None) -> Oracle:
     o = Oracle(deck)
     o.time = deck.time
     o.initial_field()
+    o.stokes_apply()  # the layer profile imposed on the fresh field (src/hst.f90:76-77)
     o.fill_all_ghosts()
     dt = deck.deltat
     istep = 0

@@ -15,9 +15,12 @@ import numpy as np
 PI = 3.141592653589793
 
 
-def kelvin(make_solver, deck):
+def kelvin(make_solver, deck, gk=0.0):
     """test_kelvin (tests/test_kelvin.f90:49-136): one Kelvin mode (ix, iz, ky) =
-    (1, 1, 1) of the linearised equations against its closed form."""
+    (1, 1, 1) of the linearised equations against its closed form.  gk: the
+    column's damping rate (the closed form gains exp(-gk t), :102-111); with
+    the hard mask (deck.damp_hard and gk > 0) the mode must vanish and the
+    errors are the absolute max |v| and |eta|."""
     deck = dataclasses.replace(deck, linear=True, time=0.0)
     s = make_solver(deck)
     ix, iz, mky = 1, 1, 1
@@ -64,6 +67,10 @@ def kelvin(make_solver, deck):
     pht = np.exp(1j * kyt * yy)
     vnum = V[1, ix, iz + deck.nz, 2:deck.ny + 2]
     etanum = 1j * kz * V[0, ix, iz + deck.nz, 2:deck.ny + 2] - 1j * kx * V[2, ix, iz + deck.nz, 2:deck.ny + 2]
+    if deck.damp_hard and gk > 0.0:
+        return float(np.abs(vnum).max()), float(np.abs(etanum).max()), n
+    vex = vex * math.exp(-gk * time)
+    etaex = etaex * math.exp(-gk * time)
     err_v = float(np.abs(vnum - vex * pht).max()) / abs(vex)
     err_eta = float(np.abs(etanum - etaex * pht).max()) / abs(etaex)
     return err_v, (err_eta if check_eta else None), n
@@ -203,3 +210,36 @@ def pressure_shear(make_solver, deck, s2=0.5):
         big = max(big, abs(B))
     return err / big
 
+
+
+def stokes_layer(make_solver, deck):
+    """test_stokes (tests/test_stokes.f90:32-77): no fluctuations; the layer is
+    prescribed for two periods, then driven by its body force and viscosity
+    alone for the rest of t_max; the (0,0) mode of w must follow the analytic
+    layer W(y, t) = A exp(-s) cos(omega (t - t0) - s), s = sqrt((eta/delta)^2 +
+    0.01), eta = y - ly/2 (src/hst_stokes.f90:6-8), and everything else must
+    stay zero.  Returns (max error / max |W|, max of everything else)."""
+    deck = dataclasses.replace(deck, time=0.0)
+    s = make_solver(deck)
+    s.time = 0.0
+    s.set_V(np.zeros((3, deck.nx + 1, 2 * deck.nz + 1, deck.ny + 4), dtype=np.complex128))
+    s.stokes_apply()
+    s.fill_all_ghosts()
+    dt = deck.deltat if deck.deltat > 0.0 else 5.0e-3
+    n = int(round(deck.t_max / dt))
+    for _ in range(n):
+        s.timestep(dt)
+    V = s.V[..., 2:deck.ny + 2]
+    yy = s.y[2:deck.ny + 2]
+    t = s.time
+    sv = np.sqrt(((yy - 0.5 * deck.ly) / deck.sl_delta) ** 2 + 0.1 ** 2)
+    ramp = 1.0 if t >= deck.sl_start else 0.0
+    wex = ramp * deck.sl_amplitude * np.exp(-sv) * np.cos(2 * PI / deck.sl_period * (t - deck.sl_start) - sv)
+    nz = deck.nz
+    err = float(np.abs(V[2, 0, nz] - wex).max())
+    other = max(float(np.abs(V[2, 0, nz].imag).max()), float(np.abs(V[0, 0, nz]).max()),
+                float(np.abs(V[1, 0, nz]).max()))
+    V = V.copy()
+    V[2, 0, nz] = 0.0
+    other = max(other, float(np.abs(V).max()))
+    return err / float(np.abs(wex).max()), other

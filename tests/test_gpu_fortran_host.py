@@ -26,7 +26,7 @@ def run_hst(tmp_path, name, precision="fp64"):
     return r
 
 
-@pytest.mark.parametrize("name", ["small", "small_s2x"])
+@pytest.mark.parametrize("name", ["small", "small_s2x", "small_stokes"])
 def test_regression_decks(tmp_path, name):
     run_hst(tmp_path, name)
     out = tmp_path / "Dati.cart.out"
@@ -39,8 +39,13 @@ def test_regression_decks(tmp_path, name):
     ia, ib = a.find(b"Vfield=\n"), b.find(b"Vfield=\n")
     assert a[:ia] == b[:ib]
     rt = np.loadtxt(tmp_path / "statistics" / "Runtimedata")
-    assert rt.shape == (11, 13)  # t = 0 and every dt_stat = 0.01 up to 0.1
-    assert abs(rt[-1, 0] - tr) < 1e-12
+    if name == "small_stokes":  # t_max = 0.025: t = 0, 0.01, 0.02; the region line beside it
+        assert rt.shape == (3, 13)
+        sl = np.loadtxt(tmp_path / "statistics" / "stokes_runtime.dat")
+        assert sl.shape == (3, 5) and np.array_equal(sl[:, 0], rt[:, 0]) and (sl[:, 1:] > 0).all()
+    else:
+        assert rt.shape == (11, 13)  # t = 0 and every dt_stat = 0.01 up to 0.1
+        assert abs(rt[-1, 0] - tr) < 1e-12
 
 
 def test_mixed_precision_driver(tmp_path):

@@ -60,12 +60,22 @@ def test_initial_energy_is_the_field_energy():
         assert abs(e - cplio.energy(hst.initial_field(d))) <= 1e-12 * e
 
 
-def test_rejects_out_of_scope_decks():
+def test_rejects_inconsistent_decks():
+    """The deck checks of read_input (src/hst_input.f90:67-82) in the library."""
     import dataclasses
 
-    d = dataclasses.replace(read_input(deck_path("small")), sl_amplitude=0.5)
-    with pytest.raises(hst.HstError, match="Stokes"):
-        hst.grid_setup(d)
+    d = read_input(deck_path("small"))
+    bad = [
+        (dict(sl_amplitude=0.5, s2_mean=0.3), "Stokes layer and the spanwise shear"),
+        (dict(sl_amplitude=0.5, sl_period=0.0), "sl_period"),
+        (dict(damp_kpar=1.0), "both be positive"),
+        (dict(damp_kpar=1.0, damp_kperp=1.0, damp_norm=0.5), "damp_norm"),
+        (dict(damp_kpar=1.0, damp_kperp=1.0, damp_edge=1.0), "damp_edge"),
+    ]
+    for kw, msg in bad:
+        with pytest.raises(hst.HstError, match=msg):
+            hst.grid_setup(dataclasses.replace(d, **kw))
+    hst.grid_setup(dataclasses.replace(d, sl_amplitude=0.5, sl_delta=0.05))  # the layer itself is accepted
 
 
 def test_benchmark_decks_have_unit_rms_velocity():

@@ -18,10 +18,11 @@ def O(oracle_built):
     return oracle_built
 
 
-@pytest.mark.parametrize("name", ["small", "small_s2x"])
+@pytest.mark.parametrize("name", ["small", "small_s2x", "small_stokes"])
 def test_golden_fields(O, name):
     """tests/regression.sh:13-28: 50 steps vs tests/reference/<deck>.fld at 1e-10
-    (compare_fields.py's max-norm); small = default shift, small_s2x = exact_shift + S2."""
+    (compare_fields.py's max-norm); small = default shift, small_s2x = exact_shift + S2,
+    small_stokes = the Stokes layer with its body force on the stretched grid."""
     d = read_input(deck_path(name))
     o = O.run_deck(d)
     t, Vr, _ = cplio.read_field(f"{GOLDEN}/reference/{name}.fld")
@@ -291,3 +292,117 @@ def test_column_energy_sums_to_the_energy(O):
     want = np.zeros_like(E2)
     want[2, -3 + d.nz] = abs(a) ** 2 * d.ly
     assert np.abs(E2 - want).max() <= 1e-13
+
+
+
+# the damping rate of the Kelvin column (1, 1) in the damping decks: with S2 = 0
+# the region is aligned with x (theta = 0), k_par = alfa0 = 2 pi/3, k_perp = beta0
+# = 2 pi, so r = sqrt((k_par/damp_kpar)^2 + (k_perp/damp_kperp)^2)
+DAMPED = {
+    "kelvin_damped": 2.0,    # semi-axes (pi, 3 pi): r = 2 sqrt(2)/3 < 1, the full rate 2
+    "kelvin_hard": 1.0,      # the same region, hard: masked (the table holds 1)
+    "kelvin_outside": 0.0,   # semi-axes (2 pi/3, 2 pi): r = sqrt(2) > 1, undamped
+}
+
+
+@pytest.mark.parametrize("name", list(DAMPED))
+def test_kelvin_damping(O, name):
+    """test_kelvin with the damping decks (tests/test_kelvin.f90:102-127): the
+    closed form times exp(-g t) at 1e-3; the hard mask leaves zero (1e-12)."""
+    d = read_input(deck_path(name))
+    gk = DAMPED[name]
+    made = []
+
+    def make(deck):
+        o = O.Oracle(deck)
+        made.append(o)
+        return o
+
+    err_v, err_eta, n = closed_forms.kelvin(make, d, gk)
+    assert np.isclose(made[-1].gdamp()[1, 1 + d.nz], gk, rtol=0, atol=1e-15)
+    if name == "kelvin_hard":
+        assert err_v <= 1e-12 and err_eta <= 1e-12, (err_v, err_eta)
+    else:
+        assert err_v <= 1e-3 and err_eta <= 1e-3, (err_v, err_eta)
+        if gk > 0.0:  # the damping is seen: the undamped closed form misses by exp(-g t) - 1
+            e0, _, _ = closed_forms.kelvin(lambda deck: O.Oracle(deck), dataclasses.replace(d, damp_rate=1e-300), 0.0)
+            assert e0 <= 1e-3
+
+
+def test_damping_table(O):
+    """damping_update (src/hst_damping.f90:78-115) against its definition at
+    chosen columns: inside the ellipse the rate, outside zero, the (0,0) column
+    only with damp_mean, the smoothstep over the last damp_edge of r, and the
+    region turned by atan2(S2, S) with damp_follow."""
+    d = dataclasses.replace(read_input(deck_path("kelvin_damped")), damp_edge=0.5)
+    o = O.Oracle(d)
+    o.outstats()  # builds the table at the start time
+    g = o.gdamp()
+    nz = d.nz
+    r11 = np.sqrt((d.alfa0 / d.damp_kpar) ** 2 + (d.beta0 / d.damp_kperp) ** 2)  # 0.943: in the edge band
+    x = (1 - r11) / 0.5
+    assert abs(g[1, 1 + nz] - 2.0 * x * x * (3 - 2 * x)) <= 1e-15
+    assert g[1, -1 + nz] == g[1, 1 + nz]           # |k_perp| is symmetric with S2 = 0
+    assert g[0, nz] == 0.0                          # the mean profiles: not without damp_mean
+    assert g[2, nz] == 0.0                          # r = 4/3
+    assert g[0, 1 + nz] == 2.0 * (lambda x: x * x * (3 - 2 * x))((1 - 2 / 3) / 0.5)  # r = 2/3: edge band too
+    o2 = O.Oracle(dataclasses.replace(d, damp_edge=0.0, damp_mean=True))
+    o2.outstats()
+    assert o2.gdamp()[0, nz] == 2.0 and o2.gdamp()[1, 1 + nz] == 2.0
+    # S2 = S: the region turns by 45 degrees; then (1, 1) and (1, -1) differ
+    o3 = O.Oracle(dataclasses.replace(d, damp_edge=0.0, s2_mean=1.0, damp_kpar=10.0, damp_kperp=4.0))
+    o3.outstats()
+    g3 = o3.gdamp()
+    c = s_ = np.sqrt(0.5)
+    for iz in (1, -1):
+        kx, kz = d.alfa0, d.beta0 * iz
+        r = np.sqrt(((kx * c + kz * s_) / 10.0) ** 2 + ((-kx * s_ + kz * c) / 4.0) ** 2)
+        assert g3[1, iz + nz] == (2.0 if r < 1 - 1e-6 else 0.0), (iz, r)
+    assert g3[1, 1 + nz] != g3[1, -1 + nz]
+    assert abs(o3.outstats_extra()[4] - 45.0) <= 1e-12
+
+
+def test_damping_statistics(O):
+    """damping_runtime.dat (src/hst_stats.f90:278-283): with the whole field in
+    the damped column (1, 1), half its energy is ly x the outstats energy and
+    half the lost power is 2 g times that."""
+    d = read_input(deck_path("kelvin_damped"))
+    o = O.Oracle(d)
+    y = o.y
+    o.field(2)[1, 1 + d.nz, :] = (0.3 - 0.1j) * np.exp(2j * np.pi * y / d.ly)
+    o.field(1)[1, 1 + d.nz, :] = 0.2 * np.exp(2j * np.pi * y / d.ly)
+    st = o.outstats()
+    ex = o.outstats_extra()
+    assert ex[5] == (o.gdamp() > 0).sum() and ex[5] > 0
+    assert abs(ex[6] - d.ly * st[0]) <= 1e-13 * ex[6]
+    assert abs(ex[7] - 2 * 2.0 * ex[6]) <= 1e-13 * ex[7]
+
+
+def test_stokes_layer(O):
+    """test_stokes (tests/test_stokes.f90:77): 1e-4 relative (2.3e-5 in the
+    reference), everything else zero (1e-12)."""
+    err, other = closed_forms.stokes_layer(O.Oracle, read_input(deck_path("stokes")))
+    assert err <= 1e-4 and other <= 1e-12, (err, other)
+
+
+def test_stokes_layer_statistics(O):
+    """stokes_runtime.dat (src/hst_stats.f90:270-277): with the layer the (0,0)
+    column is left out of the statistics (inc00 = 0); the region energies and
+    dissipations weighted by their heights add up to the totals."""
+    d = read_input(deck_path("small_stokes"))
+    o = O.Oracle(d)
+    o.initial_field()
+    o.stokes_apply()
+    o.fill_all_ghosts()
+    st = o.outstats()
+    ex = o.outstats_extra()
+    inl = np.abs(o.y[2:-2] - 0.5 * d.ly) < 8 * d.sl_delta
+    l_in, l_out = o.dyl[inl].sum(), o.dyl[~inl].sum()
+    assert 0 < inl.sum() < d.ny
+    assert abs(ex[0] * l_out + ex[1] * l_in - st[0] * d.ly) <= 1e-12 * st[0] * d.ly
+    assert abs(ex[2] * l_out + ex[3] * l_in - st[1] * d.ly) <= 1e-12 * st[1] * d.ly
+    # the imposed profile is mean flow: the energy is that of the field without it
+    o2 = O.Oracle(d)
+    o2.initial_field()
+    o2.fill_all_ghosts()
+    assert abs(o2.outstats()[0] - st[0]) <= 1e-13 * st[0]
