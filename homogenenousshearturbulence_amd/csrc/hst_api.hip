@@ -786,6 +786,20 @@ int hst_gpu_damping_mask(void) {
   });
 }
 
+int hst_gpu_project_divergence(double time) {
+  if (int rc = check_live()) return rc;
+  double sx, sz;
+  shear_shifts(time, sx, sz);
+  double2 *f[3] = {S.F[HST_V1], S.F[HST_V3], S.F[HST_V2]};
+  int rc = ghosts(3, f, sx, sz);
+  if (rc) return rc;
+  rc = run("project", 4.0 * bytes_field(), [&] {
+    return launch_project(S.D, S.F[HST_V1], S.F[HST_V2], S.F[HST_V3], S.d_der, S.stream);
+  });
+  if (rc) return rc;
+  return ghosts(3, f, sx, sz);
+}
+
 int hst_gpu_stokes_apply(double time) {
   if (int rc = check_live()) return rc;
   return stokes_apply(time);

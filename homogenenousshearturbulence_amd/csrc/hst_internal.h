@@ -176,6 +176,9 @@ int launch_pressure_rhs(const Dims &D, int mixed, const void *P0, const double2 
                         double s2now, hipStream_t s);
 int launch_pressure_mean(const Dims &D, const double2 *R, double2 *p, const double *dyl, double ly, hipStream_t s);
 
+// project_restart's divergence repair of u, v, w (interior rows; ghosts filled before).
+int launch_project(const Dims &D, double2 *V1, double2 *V2, double2 *V3, const double *der, hipStream_t s);
+
 // Layout conversion between the reference layout (host) and the device layout.
 int launch_to_device_layout(const Dims &D, const double2 *ref, double2 *dev, hipStream_t s);
 int launch_to_ref_layout(const Dims &D, const double2 *dev, double2 *ref, hipStream_t s);

@@ -1170,6 +1170,59 @@ int launch_pressure_mean(const Dims &D, const double2 *R, double2 *p, const doub
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ restart repair
+
+// project_restart (src/hst_io.f90:213-250) of one column per lane: the
+// discrete divergence i alfa D0 u + D1 v + i beta D0 w of each interior row
+// taken off u (ix > 0) or w (ix = 0, iz != 0), v = 0 in the (0,0) column.
+// Every row's divergence is that of the field before the repair: the lane
+// keeps the original rows in a five-row window while it overwrites its column.
+__global__ void __launch_bounds__(256)
+    k_project(Dims D, double2 *__restrict__ V1, double2 *__restrict__ V2, double2 *__restrict__ V3,
+              const double *__restrict__ der) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= D.ncol) return;
+  int ix, iz;
+  col_ix_iz(D, col, ix, iz);
+  const size_t nc = D.ncol;
+  const double al = D.alfa0 * ix, be = D.beta0 * iz;
+  if (ix == 0 && iz == 0) {
+    for (int iy = 0; iy < D.ny; iy++) V2[(size_t)(iy + 2) * nc + col] = make_double2(0.0, 0.0);
+    return;
+  }
+  C u[5], v[5], w[5];
+  for (int k = 1; k < 5; k++) {
+    const size_t r = (size_t)(k - 1) * nc + col;  // rows -2..1
+    u[k] = ld(V1 + r); v[k] = ld(V2 + r); w[k] = ld(V3 + r);
+  }
+  for (int iy = 0; iy < D.ny; iy++) {
+    for (int k = 0; k < 4; k++) { u[k] = u[k + 1]; v[k] = v[k + 1]; w[k] = w[k + 1]; }
+    const size_t r = (size_t)(iy + 4) * nc + col;
+    u[4] = ld(V1 + r); v[4] = ld(V2 + r); w[4] = ld(V3 + r);
+    C du{0, 0}, dv{0, 0}, dw{0, 0};
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+      du = du + DER(der, iy, 0, j - 2) * u[j];
+      dv = dv + DER(der, iy, 1, j - 2) * v[j];
+      dw = dw + DER(der, iy, 0, j - 2) * w[j];
+    }
+    const size_t o = (size_t)(iy + 2) * nc + col;
+    if (ix != 0) {
+      const C div = dv + imul(be, dw) + imul(al, du);
+      // u - div / (i alfa) = (u.x - div.y / alfa, u.y + div.x / alfa)
+      st(V1 + o, C{u[2].x - div.y / al, u[2].y + div.x / al});
+    } else {
+      const C div = dv + imul(be, dw);
+      st(V3 + o, C{w[2].x - div.y / be, w[2].y + div.x / be});
+    }
+  }
+}
+
+int launch_project(const Dims &D, double2 *V1, double2 *V2, double2 *V3, const double *der, hipStream_t s) {
+  hipLaunchKernelGGL(k_project, dim3((D.ncol + 255) / 256), dim3(256), 0, s, D, V1, V2, V3, der);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ layouts
 
 // reference layout (col-major f(iy, iz, ix): row fastest) <-> device [row][col]

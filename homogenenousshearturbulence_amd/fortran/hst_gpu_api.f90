@@ -143,6 +143,10 @@ module hst_gpu_api
       import :: c_int, c_double
       real(c_double), value :: time
     end function
+    integer(c_int) function hst_gpu_project_divergence(time) bind(c, name='hst_gpu_project_divergence')
+      import :: c_int, c_double
+      real(c_double), value :: time
+    end function
     integer(c_int) function hst_gpu_damping_mask() bind(c, name='hst_gpu_damping_mask')
       import :: c_int
     end function

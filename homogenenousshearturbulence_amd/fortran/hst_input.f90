@@ -64,8 +64,8 @@ contains
       print *, 'ERROR: one GPU holds every y row: npy must be 0 or 1'
       error stop 1
     end if
-    if (trim(restart_remap) /= 'strict') then
-      print *, "ERROR: restart_remap must be 'strict' (a mode-index remap is not part of this build)"
+    if (trim(restart_remap) /= 'strict' .and. trim(restart_remap) /= 'mode_index') then
+      print *, "ERROR: restart_remap must be 'strict' or 'mode_index'"
       error stop 1
     end if
     if (sl_amplitude /= 0.0d0 .and. (s2_amplitude /= 0.0d0 .or. s2_mean /= 0.0d0)) then

@@ -34,8 +34,13 @@ contains
     end do
   end subroutine download_velocity
 
-  ! Reads filename into V (same mesh), or generates the initial field when the
-  ! file is absent; the device copy is updated, the caller fills the ghosts.
+  ! Reads filename into V, or generates the initial field when the file is
+  ! absent; the device copy is updated, the caller fills the ghosts
+  ! (restart_read, src/hst_io.f90:63-190).  A file of another resolution is
+  ! remapped: the Fourier modes present in both fields are copied (x and z are
+  ! mode indices), y is resampled by the nearest row of the file's uniform
+  ! grid.  Other alpha0/beta0 need restart_remap = 'mode_index', which then
+  ! repairs the discrete divergence on the device (project_restart).
   subroutine restart_read(filename)
     character(len=*), intent(in) :: filename
     integer :: io, unit, p
@@ -66,14 +71,23 @@ contains
     r_beta0 = header_value(head, 'beta0=')
     r_re = header_value(head, 'Re=')
     source_ny = r_nz - 1
-    if (r_nx /= nx .or. r_ny /= nz .or. source_ny /= ny) then
-      print *, 'ERROR: the restart mesh differs from the deck (a remap is not part of this build)'
+    if (r_nx < 0 .or. r_ny < 0 .or. source_ny < 1) then
+      print *, 'ERROR: invalid mesh dimensions in '//trim(filename)
       error stop 1
+    end if
+    if (has_terminal .and. (r_nx /= nx .or. r_ny /= nz .or. source_ny /= ny)) then
+      print *, '   interpolating restart mesh: file (nx, nz, ny) = ', r_nx, r_ny, source_ny
+      print *, '                              deck (nx, nz, ny) = ', nx, nz, ny
     end if
     if (abs(r_alfa0 - alfa0) > 1.0d-4*max(abs(alfa0), 1.0d0) .or. abs(r_beta0 - beta0) > 1.0d-4*max(abs(beta0), 1.0d0)) &
       then
-      print *, 'ERROR: restart alpha0/beta0 do not match the deck:', r_alfa0, r_beta0, alfa0, beta0
-      error stop 1
+      if (trim(restart_remap) == 'mode_index') then
+        if (has_terminal) print *, '   restart_remap=mode_index: remapping signed Fourier indices and projecting' // &
+          ' on target wavenumbers'
+      else
+        print *, 'ERROR: restart alpha0/beta0 do not match the deck:', r_alfa0, r_beta0, alfa0, beta0
+        error stop 1
+      end if
     end if
     p = index(head, 'time='//LF)
     read (unit, pos=p + 6) r_time
@@ -81,15 +95,17 @@ contains
     read (unit, pos=p + 4) r_S
     if (has_terminal .and. (abs(r_re - re) > 1.0d-9*re .or. r_S /= S)) print *, '   note: Re or S differ from the file'
     if (time_from_restart) time = r_time
-    allocate (buf(3, -2:ny + 1, -nz:nz, 0:nx))
+    ! the file holds x = 0..r_nx, z = -r_ny..r_ny and the rows -2..source_ny+1
+    allocate (buf(3, -2:source_ny + 1, -r_ny:r_ny, 0:r_nx))
     p = index(head, 'Vfield='//LF)
     read (unit, pos=p + 8) buf
     close (unit)
-    ! the CPL shear grid is uniform: nearest row by coordinate (src/hst_io.f90:155-167)
+    ! the CPL shear grid is uniform whatever the target's stretching: the
+    ! nearest row by coordinate, the end point wrapped (src/hst_io.f90:155-167)
     source_dy = ly/real(source_ny, C_DOUBLE)
     V = 0
-    do ix = 0, nx
-      do iz = -nz, nz
+    do ix = 0, min(nx, r_nx)
+      do iz = max(-nz, -r_ny), min(nz, r_ny)
         do iy = 0, ny - 1
           src_iy = nint(modulo(y(iy), ly)/source_dy)
           if (src_iy == source_ny) src_iy = 0
@@ -99,6 +115,10 @@ contains
     end do
     deallocate (buf)
     call upload_velocity()
+    if (trim(restart_remap) == 'mode_index') then
+      call gpu_check(hst_gpu_project_divergence(time), 'project_restart')
+      call download_velocity()
+    end if
   end subroutine restart_read
 
   ! Writes the device velocity with the CPL header.

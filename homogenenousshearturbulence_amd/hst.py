@@ -34,7 +34,7 @@ EXPORTS = (
     "hst_gpu_buildrhs_prepare", "hst_gpu_buildrhs", "hst_gpu_shear_shift", "hst_gpu_line_solve", "hst_gpu_linsolve",
     "hst_gpu_timestep", "hst_gpu_outstats", "hst_gpu_set_chunk_rows", "hst_gpu_memory", "hst_gpu_profile",
     "hst_gpu_profile_count", "hst_gpu_profile_entry", "hst_gpu_pressure", "hst_gpu_spectrum",
-    "hst_gpu_stokes_apply", "hst_gpu_damping_mask", "hst_gpu_outstats_extra",
+    "hst_gpu_stokes_apply", "hst_gpu_damping_mask", "hst_gpu_outstats_extra", "hst_gpu_project_divergence",
 )
 
 
@@ -86,6 +86,7 @@ def lib() -> ctypes.CDLL:
         "hst_gpu_pressure": [d, i],
         "hst_gpu_spectrum": [vp, ctypes.POINTER(ctypes.c_int32)],
         "hst_gpu_stokes_apply": [d],
+        "hst_gpu_project_divergence": [d],
         "hst_gpu_damping_mask": [],
         "hst_gpu_outstats_extra": [dp],
     }
@@ -300,6 +301,10 @@ class Solver:
     def stokes_apply(self) -> None:
         """stokes_apply at self.time (src/hst_stokes.f90:103-118), ghosts included."""
         _check(lib().hst_gpu_stokes_apply(self.time), "hst_gpu_stokes_apply")
+
+    def project_divergence(self) -> None:
+        """project_restart at self.time (src/hst_io.f90:213-250), ghosts filled before and after."""
+        _check(lib().hst_gpu_project_divergence(self.time), "hst_gpu_project_divergence")
 
     def damping_mask(self) -> None:
         _check(lib().hst_gpu_damping_mask(), "hst_gpu_damping_mask")

@@ -165,6 +165,14 @@ int hst_gpu_outstats(double time, double out[10]);
  * the layer. */
 int hst_gpu_stokes_apply(double time);
 
+/* project_restart (src/hst_io.f90:213-250) with the ghost fills around it
+ * (:184-190), at `time`: after a restart_remap = 'mode_index' load, the
+ * discrete divergence i alfa D0 u + D1 v + i beta D0 w of every interior row
+ * is taken off u (ix > 0) or w (ix = 0), and v = 0 in the (0,0) column.  Each
+ * row's divergence is that of the loaded field (the reference's loop updates
+ * in place while neighbouring rows read; here no row sees another's update). */
+int hst_gpu_project_divergence(double time);
+
 /* damping_mask (src/hst_damping.f90:119-133): the right-hand sides and
  * carried terms of the hard-masked columns set to zero (for a host that runs
  * the pieces of a substep; hst_gpu_timestep does it inside buildrhs).  The

@@ -1182,6 +1182,47 @@ void hsto_outstats_at(hsto *o, double time, double out[10]) {
 
 void hsto_outstats_extra(const hsto *o, double out[8]) { memcpy(out, o->extra, sizeof(double) * 8); }
 
+/* project_restart (src/hst_io.f90:213-250) with the ghost fills around it
+ * (:184-190) at `time`.  The reference updates V in place while the
+ * stencils of neighbouring rows read it; here every row's divergence is that
+ * of the field before the repair (the originals are copied first). */
+void hsto_project_divergence(hsto *o, double time) {
+  fill_ghosts_t(o, 1, time);
+  fill_ghosts_t(o, 3, time);
+  fill_ghosts_t(o, 2, time);
+  const size_t nf = (size_t)o->NXm * o->NZm * o->NYg;
+  cplx *U = (cplx *)malloc(sizeof(cplx) * nf * 3);
+  memcpy(U, o->f[1], sizeof(cplx) * nf);
+  memcpy(U + nf, o->f[2], sizeof(cplx) * nf);
+  memcpy(U + 2 * nf, o->f[3], sizeof(cplx) * nf);
+  for (int ix = 0; ix <= o->nx; ix++)
+    for (int iz = -o->nz; iz <= o->nz; iz++) {
+      const double al = o->d.alfa0 * ix, be = o->d.beta0 * iz;
+      for (int iy = 0; iy < o->ny; iy++) {
+        size_t c = FIDX(o, ix, iz, iy);
+        cplx du = 0.0, dv = 0.0, dw = 0.0;
+        for (int j = -2; j <= 2; j++) {
+          du = du + DER(o, iy, 0, j) * U[c + j];
+          dv = dv + DER(o, iy, 1, j) * U[nf + c + j];
+          dw = dw + DER(o, iy, 0, j) * U[2 * nf + c + j];
+        }
+        if (ix != 0) {
+          cplx div = dv + I * be * dw + I * al * du;
+          o->f[1][c] = (creal(U[c]) - cimag(div) / al) + I * (cimag(U[c]) + creal(div) / al);
+        } else if (iz != 0) {
+          cplx div = dv + I * be * dw;
+          o->f[3][c] = (creal(U[2 * nf + c]) - cimag(div) / be) + I * (cimag(U[2 * nf + c]) + creal(div) / be);
+        } else {
+          o->f[2][c] = 0.0;
+        }
+      }
+    }
+  free(U);
+  fill_ghosts_t(o, 1, time);
+  fill_ghosts_t(o, 3, time);
+  fill_ghosts_t(o, 2, time);
+}
+
 /* gdamp(-nz:nz, 0:nx) as last built (zero before the first damping_update) */
 double *hsto_gdamp(hsto *o) { return o->gdamp; }
 

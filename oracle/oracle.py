@@ -64,6 +64,7 @@ def lib():
         L.hsto_max_threads.restype = i
         L.hsto_pressure.argtypes = [vp, d, i]
         L.hsto_stokes_apply.argtypes = [vp, d]
+        L.hsto_project_divergence.argtypes = [vp, d]
         L.hsto_damping_mask.argtypes = [vp]
         L.hsto_outstats_extra.argtypes = [vp, vp]
         L.hsto_gdamp.restype = vp
@@ -174,6 +175,9 @@ class Oracle:
     def stokes_apply(self) -> None:
         lib().hsto_stokes_apply(self.h, self.time)
 
+    def project_divergence(self) -> None:
+        lib().hsto_project_divergence(self.h, self.time)
+
     def damping_mask(self) -> None:
         lib().hsto_damping_mask(self.h)
 
@@ -198,6 +202,23 @@ class Oracle:
         out = np.zeros(10)
         lib().hsto_outstats_at(self.h, self.time, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         return out
+
+
+def remap_field(Vf: np.ndarray, sizes: dict, deck: Deck, y: np.ndarray) -> np.ndarray:
+    """restart_read's remap of a file field (src/hst_io.f90:155-167): the
+    Fourier modes present in both copied, every target row from the file row
+    nearest to it on the file's uniform grid (the end point wrapped); ghost
+    rows zero.  Vf: the file's field as cplio.read_field returns it."""
+    nx_s, nz_s, ny_s = sizes["nx"], sizes["nz"], sizes["ny"]
+    V = np.zeros((3, deck.nx + 1, 2 * deck.nz + 1, deck.ny + 4), dtype=np.complex128)
+    dy_s = deck.ly / ny_s
+    nxc, nzc = min(deck.nx, nx_s), min(deck.nz, nz_s)
+    for iy in range(deck.ny):
+        src = int(np.floor(np.mod(y[iy + 2], deck.ly) / dy_s + 0.5))  # nint of a non-negative value
+        if src == ny_s:
+            src = 0
+        V[:, :nxc + 1, deck.nz - nzc:deck.nz + nzc + 1, iy + 2] = Vf[:, :nxc + 1, nz_s - nzc:nz_s + nzc + 1, src + 2]
+    return V
 
 
 def uniform_from_key(seed, component, iy, iz, ix) -> float:

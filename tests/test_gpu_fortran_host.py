@@ -113,3 +113,47 @@ def test_snapshots_pressure_and_spectrum(tmp_path, oracle_built):
         assert sel.sum() == nsamp
         assert abs(E[:, 4].sum() - d.ly * rt[sel, 9].mean()) <= 1e-6 * E[:, 4].sum()  # 8 printed digits
         t_prev = t
+
+
+@pytest.mark.parametrize("remap", ["strict", "mode_index"])
+def test_restart_remap(tmp_path, oracle_built, remap):
+    """restart_read of a file of another resolution (src/hst_io.f90:63-190):
+    the common Fourier modes copied, the nearest file row per target row of a
+    stretched grid; with restart_remap = 'mode_index' and another alpha0 the
+    divergence repair (project_restart) follows.  t_max = 0: the driver writes
+    the loaded field back as Dati.cart.out, compared with the oracle's remap
+    (exactly) and repair (1e-12)."""
+    from oracle import oracle as O
+
+    from homogenenousshearturbulence_amd.deck import read_input
+
+    src = str(tmp_path / "src.fld")
+    shutil.copy(f"{GOLDEN}/reference/small.fld", src)
+    text = open(deck_path("small")).read()
+    text = text.replace("nx = 15, ny = 32, nz = 15", "nx = 21, ny = 45, nz = 11")
+    text = text.replace("ly = 2.0 /", "ly = 2.0, ystretch = 1.2 /")
+    text = text.replace("t_max = 0.1", "t_max = 0.0").replace("dt_stat = 0.01", "dt_stat = 1e9")
+    text = text.replace("time_from_restart = .false.", "time_from_restart = .true., restart_file = 'src.fld', "
+                        f"restart_remap = '{remap}'")
+    if remap == "mode_index":
+        text = text.replace("alfa0 = 2.0943951023931953", "alfa0 = 1.5")
+    (tmp_path / "hst.in").write_text(text)
+    d = read_input(str(tmp_path / "hst.in"))
+    assert (d.nx, d.ny, d.nz, d.ystretch) == (21, 45, 11, 1.2)
+    r = subprocess.run([HST, "hst.in"], cwd=tmp_path, capture_output=True, text=True, timeout=150)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "interpolating restart mesh" in r.stdout
+    t, V, sizes = cplio.read_field(str(tmp_path / "Dati.cart.out"))
+    tf, Vf, sf = cplio.read_field(src)
+    assert t == tf and sizes == {"nx": 21, "nz": 11, "ny": 45}
+    o = O.Oracle(d)
+    want = O.remap_field(Vf, sf, d, o.y)
+    if remap == "strict":
+        assert np.array_equal(V[..., 2:-2], want[..., 2:-2])
+    else:
+        o.set_V(want)
+        o.time = t
+        o.project_divergence()
+        err = np.abs(V[..., 2:-2] - o.V[..., 2:-2]).max() / np.abs(o.V).max()
+        assert err <= 1e-12, err
+        assert np.abs(V[..., 2:-2] - want[..., 2:-2]).max() > 1e-6

@@ -406,3 +406,62 @@ def test_stokes_layer_statistics(O):
     o2.initial_field()
     o2.fill_all_ghosts()
     assert abs(o2.outstats()[0] - st[0]) <= 1e-13 * st[0]
+
+
+def test_remap_nearest_row(O):
+    """restart_read's y resampling (src/hst_io.f90:155-167) against brute force:
+    every target row of a stretched grid takes the file row at the smallest
+    periodic distance; the modes outside either field stay zero."""
+    src = dataclasses.replace(read_input(deck_path("small")), nx=7, nz=5, ny=20)
+    tgt = dataclasses.replace(read_input(deck_path("small")), nx=9, nz=3, ny=27, ystretch=1.5)
+    rng = np.random.default_rng(1)
+    Vf = rng.standard_normal((3, src.nx + 1, 2 * src.nz + 1, src.ny + 4)) + 0j
+    Vf[..., 2:-2] += np.arange(src.ny)[None, None, None, :] * 10.0  # the row index is readable
+    o = O.Oracle(tgt)
+    V = O.remap_field(Vf, {"nx": src.nx, "nz": src.nz, "ny": src.ny}, tgt, o.y)
+    ys = np.arange(src.ny) * src.ly / src.ny
+    for iy in range(tgt.ny):
+        d = np.abs(np.mod(o.y[iy + 2], tgt.ly) - ys)
+        d = np.minimum(d, tgt.ly - d)
+        best = int(np.argmin(d))
+        assert np.array_equal(V[:, :src.nx + 1, :, iy + 2], Vf[:, :, src.nz - tgt.nz:src.nz + tgt.nz + 1, best + 2])
+    assert not V[:, src.nx + 1:].any()
+    assert not V[..., :2].any() and not V[..., -2:].any()
+
+
+def test_project_divergence_symbol(O):
+    """project_restart (src/hst_io.f90:213-250) on single y modes of a uniform
+    grid, where D0 and D1 act as their Fourier symbols d0, d1: a column with
+    divergence div = i alfa d0 u + d1 v + i beta d0 w leaves with div (1 - d0)
+    (u changed for ix > 0, w for ix = 0), and v of the (0,0) column is zero."""
+    d = read_input(deck_path("small"))
+    o = O.Oracle(d)
+    o.time = 0.0
+    m = 3
+    ky = 2 * np.pi * m / d.ly
+    yy = o.y
+    amps = {(2, -1): (0.3 + 0.1j, -0.2j, 0.5), (0, 2): (0.1, 0.4 - 0.3j, 0.2 + 0.2j), (0, 0): (0.3, 0.7, 0.1)}
+    for (ix, iz), (a, b, c) in amps.items():
+        o.field(1)[ix, iz + d.nz] = a * np.exp(1j * ky * yy)
+        o.field(2)[ix, iz + d.nz] = b * np.exp(1j * ky * yy)
+        o.field(3)[ix, iz + d.nz] = c * np.exp(1j * ky * yy)
+    o.project_divergence()
+    dy = d.ly / d.ny
+    der = o.der  # der[iy, n, j + 2]
+    d0 = (der[:, 0, :] * np.exp(1j * ky * dy * np.arange(-2, 3))[None, :]).sum(axis=1)
+    d1 = (der[:, 1, :] * np.exp(1j * ky * dy * np.arange(-2, 3))[None, :]).sum(axis=1)
+    ph = np.exp(1j * ky * yy[2:-2])
+    for (ix, iz), (a, b, c) in amps.items():
+        al, be = d.alfa0 * ix, d.beta0 * iz
+        u, v, w = (o.field(k)[ix, iz + d.nz, 2:-2] for k in (1, 2, 3))
+        if ix == 0 and iz == 0:
+            assert not v.any() and np.allclose(u, a * ph, rtol=0, atol=1e-15)
+            continue
+        div0 = (1j * al * d0 * a + d1 * b + 1j * be * d0 * c) * ph
+        div1 = 1j * al * d0 * u + d1 * v + 1j * be * d0 * w
+        assert np.abs(div1 - div0 * (1 - d0)).max() <= 1e-13 * np.abs(div0).max()
+        assert np.abs(div0).max() > 0.1
+        if ix != 0:
+            assert np.array_equal(w, c * ph) and np.array_equal(v, b * ph)
+        else:
+            assert np.array_equal(u, a * ph) and np.array_equal(v, b * ph)
