@@ -296,9 +296,11 @@ static const void *P_of(int p) {  // product p = 0..5
   return (const char *)S.P + (size_t)p * S.D.NYg * S.D.ncol * cs;
 }
 
+// mode 0: the six products; 1: the round trip; 2: the CFL number only;
+// 3: the five combinations vv - uu, vv - ww, uv, vw, uw (a time step's own)
 static int nonlinear_pass(int mode, bool want_cfl, double *cfl) {
   const double2 *V[3] = {S.F[1], S.F[2], S.F[3]};
-  const int np = (mode == 0 ? 6 : 3);
+  const int np = (mode == 0 ? 6 : (mode == 3 ? 5 : 3));
   int npart_total = 0;
   for (int row0 = 0; row0 < S.D.NYg; row0 += S.chunk) {
     const int nrows = std::min(S.chunk, S.D.NYg - row0);
@@ -349,7 +351,7 @@ static int buildrhs(const double ode[3], double deltat, double time, int mode, b
     a.hard = (mask && S.deck.damp_hard) ? 1 : 0;
   }
   a.no_mean_vw = stokes_body() ? 1 : 0;
-  if (mode == 0 || mode == 3) {
+  if (mode == 0 || mode >= 3) {  // the modes with buildrhs_prepare
     if (int rc = stokes_force(time)) return rc;
     if (stokes_body()) a.fy = S.d_fy;
   }
@@ -361,8 +363,9 @@ static int buildrhs(const double ode[3], double deltat, double time, int mode, b
   const double ps = S.mixed ? 8.0 : 16.0, pts = (double)S.D.ny * S.D.ncol;
   double bytes = pts * (5.0 * 16.0 + 4.0 * ps);
   if (mode == 3) bytes += 6.0 * pts * ps;
+  else if (mode == 4) bytes += 5.0 * pts * ps;
   else if (mode != 0) bytes = pts * (4.0 * 16.0 + 4.0 * ps + 3.0 * ps);
-  return run(mode == 3 ? "buildrhs" : (mode == 0 ? "buildrhs_prepare" : "buildrhs_group"), bytes, [&] {
+  return run(mode >= 3 ? "buildrhs" : (mode == 0 ? "buildrhs_prepare" : "buildrhs_group"), bytes, [&] {
     return launch_buildrhs(S.D, S.mixed, a, S.F[1], S.F[2], S.F[3], P, S.F[HST_RHS1], S.F[HST_RHS2], S.F[HST_OLD1],
                            S.F[HST_OLD2], S.d_der, S.d_y, S.stream);
   });
@@ -849,17 +852,20 @@ int hst_gpu_timestep(double deltat, double *time, double *cfl) {
     const double *ode = RK[i];
     int rc = 0;
     const bool want = (i == 2) && cfl != nullptr;
-    if (!S.deck.linear) rc = nonlinear_pass(0, want, &c);
+    // the five product combinations the right-hand sides need; the six
+    // products themselves when a closure is attached (it adds its modelled
+    // stresses to them)
+    const bool six = g_hook != nullptr;
+    if (!S.deck.linear) rc = nonlinear_pass(six ? 0 : 3, want, &c);
     else if (want) rc = nonlinear_pass(2, true, &c);  // linear: the transform only for the CFL number
     if (rc) return rc;
-    // an attached closure adds its modelled stresses to the six products
-    if (!S.deck.linear && g_hook && (rc = g_hook(g_hook_ctx, t))) return rc;
+    if (!S.deck.linear && six && (rc = g_hook(g_hook_ctx, t))) return rc;
     const double dt_sub = 2.0 / ode[0] * deltat;
     const bool fuse_shift = !S.deck.exact_shift;
     const double dgx = S.deck.S * dt_sub, dgz = s2_integral(t, t + dt_sub);
     if ((rc = join_side())) return rc;  // the previous substep's oldrhs shift
     if ((rc = damping_update(t))) return rc;
-    rc = buildrhs(ode, deltat, t, S.deck.linear ? 0 : 3, fuse_shift, dgx, dgz, true);
+    rc = buildrhs(ode, deltat, t, S.deck.linear ? 0 : (six ? 3 : 4), fuse_shift, dgx, dgz, true);
     if (rc) return rc;
     if (S.deck.exact_shift && (rc = exact_shift_split(dt_sub, t))) return rc;
     t = t + dt_sub;

@@ -185,6 +185,24 @@ __global__ void __launch_bounds__((ZCfg<R, NZD, true>::NT))
   }
 }
 
+// ------------------------------------------------------------------ products
+
+// The five combinations of the products a substep needs (MODE 3 of the x
+// passes): vv - uu, vv - ww, uv, vw, uw, each times factor, on the even (x)
+// and odd (y) points of a line.  The right-hand sides depend on uu, vv, ww
+// only through these two differences (and through uv, vw, uw), so five
+// spectra replace the six of build_products.
+template <typename R>
+__device__ __forceinline__ Cx<R> combined_product(int p, const Cx<R> &u, const Cx<R> &v, const Cx<R> &w, R factor) {
+  switch (p) {
+    case 0: return Cx<R>{(v.x * v.x - u.x * u.x) * factor, (v.y * v.y - u.y * u.y) * factor};
+    case 1: return Cx<R>{(v.x * v.x - w.x * w.x) * factor, (v.y * v.y - w.y * w.y) * factor};
+    case 2: return Cx<R>{u.x * v.x * factor, u.y * v.y * factor};
+    case 3: return Cx<R>{v.x * w.x * factor, v.y * w.y * factor};
+    default: return Cx<R>{u.x * w.x * factor, u.y * w.y * factor};
+  }
+}
+
 // ------------------------------------------------------------------ k_xpass
 
 // One (row, z point) line per group of T threads, G lines per block.
@@ -245,13 +263,15 @@ __global__ void __launch_bounds__((XCfg<R, M>::NT))
   }
   if constexpr (MODE == 2) return;
   const R factor = (R)D.factor;
-  constexpr int NP = (MODE == 0 ? 6 : 3);
+  constexpr int NP = (MODE == 0 ? 6 : (MODE == 3 ? 5 : 3));
   for (int p = 0; p < NP; p++) {
     for (int m = t; m < M; m += T) {
       Cx<R> r;
       if constexpr (MODE == 1) {
         const Cx<R> a = (p == 0 ? u[m] : (p == 1 ? v[m] : w[m]));
         r = Cx<R>{a.x * factor, a.y * factor};
+      } else if constexpr (MODE == 3) {
+        r = combined_product<R>(p, u[m], v[m], w[m], factor);
       } else {
         const Cx<R> a = (p == 0 || p == 3 || p == 5) ? u[m] : (p == 1 || p == 4 ? v[m] : w[m]);
         const Cx<R> b = (p == 0) ? u[m] : (p == 1 || p == 3) ? v[m] : w[m];
@@ -415,7 +435,7 @@ __global__ void __launch_bounds__(64, (xline_min_waves<R, M>()))
   }
   if constexpr (MODE == 2) return;
   const R factor = (R)D.factor;
-  constexpr int NP = (MODE == 0 ? 6 : 3);
+  constexpr int NP = (MODE == 0 ? 6 : (MODE == 3 ? 5 : 3));
   // unrolled: with p known at compile time no register array is selected at
   // run time (a run-time select of arrays would put them in scratch memory)
 #pragma unroll
@@ -426,6 +446,8 @@ __global__ void __launch_bounds__(64, (xline_min_waves<R, M>()))
       if constexpr (MODE == 1) {
         const Cx<R> a = (p == 0 ? u[e] : (p == 1 ? v[e] : w[e]));
         z[e] = Cx<R>{a.x * factor, a.y * factor};
+      } else if constexpr (MODE == 3) {
+        z[e] = combined_product<R>(p, u[e], v[e], w[e], factor);
       } else {
         const Cx<R> a = (p == 0 || p == 3 || p == 5) ? u[e] : (p == 1 || p == 4 ? v[e] : w[e]);
         const Cx<R> b = (p == 0) ? u[e] : ((p == 1 || p == 3) ? v[e] : w[e]);
@@ -486,6 +508,7 @@ static int run_xpass(const Dims &D, int mode, const void *Hv, void *Hp, int row0
                      tb.twh, dyl, cfl_part, WC, row0)
     if (mode == 0) LX(0, want_cfl);
     else if (mode == 1) LX(1, want_cfl);
+    else if (mode == 3) LX(3, want_cfl);
     else LX(2, 1);
 #undef LX
     return hipGetLastError();
@@ -499,6 +522,9 @@ static int run_xpass(const Dims &D, int mode, const void *Hv, void *Hp, int row0
                          tb.twx, tb.twh, dyl, cfl_part, want_cfl);
     else if (mode == 1)
       hipLaunchKernelGGL((k_xpass<R, M, 1>), grid, dim3(C::NT), 0, s, (const Cx<R> *)Hv, (Cx<R> *)Hp, D, row0, nrows,
+                         tb.twx, tb.twh, dyl, cfl_part, want_cfl);
+    else if (mode == 3)
+      hipLaunchKernelGGL((k_xpass<R, M, 3>), grid, dim3(C::NT), 0, s, (const Cx<R> *)Hv, (Cx<R> *)Hp, D, row0, nrows,
                          tb.twx, tb.twh, dyl, cfl_part, want_cfl);
     else
       hipLaunchKernelGGL((k_xpass<R, M, 2>), grid, dim3(C::NT), 0, s, (const Cx<R> *)Hv, (Cx<R> *)Hp, D, row0, nrows,
@@ -570,7 +596,7 @@ double bytes_zift(const Dims &D, int mixed, int nrows) {
 }
 double bytes_xpass(const Dims &D, int mixed, int mode, int nrows) {
   const double cs = mixed ? 8.0 : 16.0, h = (double)D.NXm * D.nzd * nrows * cs;
-  return mode == 0 ? 9.0 * h : (mode == 1 ? 6.0 * h : 3.0 * h);
+  return mode == 0 ? 9.0 * h : (mode == 3 ? 8.0 * h : (mode == 1 ? 6.0 * h : 3.0 * h));
 }
 double bytes_zfft(const Dims &D, int mixed, int np, int nrows) {
   const double cs = mixed ? 8.0 : 16.0;

@@ -56,7 +56,9 @@ __device__ __forceinline__ Cx<PR> ldp(const typename PTraits<PR>::T2 *p) {
 // column over a chunk of rows, with five-row windows of u, v, w and of the six
 // product spectra in registers.  PR: the arithmetic of the product stencils.
 // MODE 3: prepare and both product groups (the fused substep); 0: prepare only;
-// 1, 2: add the products of group 1 or 2 to rhs, oldrhs as buildrhs(ODE, g).
+// 1, 2: add the products of group 1 or 2 to rhs, oldrhs as buildrhs(ODE, g);
+// 4: prepare and the five combined products of the x pass's MODE 3
+// (vv - uu, vv - ww, uv, vw, uw), the same sums regrouped.
 template <typename PR, int MODE, bool SHIFT>
 __global__ void __launch_bounds__(256)
     k_buildrhs(Dims D, RhsArgs a, const double2 *__restrict__ V1, const double2 *__restrict__ V2,
@@ -64,8 +66,8 @@ __global__ void __launch_bounds__(256)
                double2 *__restrict__ R1, double2 *__restrict__ R2, typename PTraits<PR>::T2 *__restrict__ O1,
                typename PTraits<PR>::T2 *__restrict__ O2, const double *__restrict__ der,
                const double *__restrict__ y, int rch) {
-  constexpr bool NONLIN = MODE != 0, PREP = (MODE == 0 || MODE == 3);
-  constexpr int Q0 = (MODE == 2 ? 3 : 0), Q1 = (MODE == 1 ? 3 : 6);
+  constexpr bool NONLIN = MODE != 0, PREP = (MODE == 0 || MODE == 3 || MODE == 4);
+  constexpr int Q0 = (MODE == 2 ? 3 : 0), Q1 = (MODE == 1 ? 3 : (MODE == 4 ? 5 : 6));
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= D.ncol) return;
   int ix, iz;
@@ -158,7 +160,41 @@ __global__ void __launch_bounds__(256)
     r1 = C{r1.x / a.deltat, r1.y / a.deltat} + imp1 - a.ode2 * o1old + a.ode1 * expl;
     o1 = expl;
     }
-    if constexpr (NONLIN) {
+    if constexpr (MODE == 4) {
+      // buildrhs (:169-207) summed over the six products and regrouped by
+      // the combinations P0 = vv - uu, P1 = vv - ww, P2 = uv, P3 = vw, P4 = uw:
+      //   eta: alfa beta D0 (P1 - P0) + (beta^2 - alfa^2) D0 P4 - i beta D1 P2 + i alfa D1 P3
+      //        ((0,0): (-Re D1 P2, -Re D1 P3))
+      //   d2v: alfa^2 D1 P0 + beta^2 D1 P1 - 2 alfa beta D1 P4 + i alfa (D2 + k^2 D0) P2
+      //        + i beta (D2 + k^2 D0) P3
+      Cx<PR> d0[5], d1[5], d2[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) { d0[q] = Cx<PR>{0, 0}; d1[q] = Cx<PR>{0, 0}; d2[q] = Cx<PR>{0, 0}; }
+#pragma unroll
+      for (int j = 0; j < 5; j++) {
+        const PR c0 = (PR)DER(der, iy, 0, j - 2), c1 = (PR)DER(der, iy, 1, j - 2), c2 = (PR)DER(der, iy, 2, j - 2);
+#pragma unroll
+        for (int q = 0; q < 5; q++) {  // (D2 of P0, P1, P4 is not used and drops out)
+          d0[q] = d0[q] + c0 * p[q][j];
+          d1[q] = d1[q] + c1 * p[q][j];
+          d2[q] = d2[q] + c2 * p[q][j];
+        }
+      }
+      Cx<PR> e, ex;
+      if (m00) {
+        const PR rw = a.no_mean_vw ? (PR)0 : -d1[3].x;
+        e = Cx<PR>{-d1[2].x, rw};
+      } else {
+        e = (alp * bep) * (d0[1] - d0[0]) + (bep * bep - alp * alp) * d0[4] - imul(bep, d1[2]) + imul(alp, d1[3]);
+      }
+      ex = (alp * alp) * d1[0] + (bep * bep) * d1[1] - ((PR)2 * alp * bep) * d1[4] +
+           imul(alp, d2[2] + kkp * d0[2]) + imul(bep, d2[3] + kkp * d0[3]);
+      const C exd{(double)ex.x, (double)ex.y}, ed{(double)e.x, (double)e.y};
+      r2 = r2 + a.ode1 * exd;
+      o2 = o2 + exd;
+      r1 = r1 + a.ode1 * ed;
+      o1 = o1 + ed;
+    } else if constexpr (NONLIN) {
       // buildrhs for the six products in the order uu vv ww uv vw uw (:169-207)
       PR e0[5], e1[5], e2[5];
 #pragma unroll
@@ -217,7 +253,7 @@ int launch_buildrhs(const Dims &D, int mixed, const RhsArgs &a, const double2 *V
   const int rch = 64;  // rows per lane: the 5-row windows re-read 4 rows per chunk
   dim3 grid((D.ncol + 255) / 256, (D.ny + rch - 1) / rch);
   const size_t pstride = (size_t)D.NYg * D.ncol;
-  const int mode = a.nonlinear;  // 0 prepare, 1/2 one group, 3 all
+  const int mode = a.nonlinear;  // 0 prepare, 1/2 one group, 3 all, 4 all from the five combinations
   const bool sh = a.shift != 0;
 #define L(PR, MD, SH)                                                                                            \
   hipLaunchKernelGGL((k_buildrhs<PR, MD, SH>), grid, dim3(256), 0, s, D, a, V1, V2, V3,                          \
@@ -228,6 +264,7 @@ int launch_buildrhs(const Dims &D, int mixed, const RhsArgs &a, const double2 *V
     case 0: if (sh) L(PR, 0, true); else L(PR, 0, false); break;   \
     case 1: L(PR, 1, false); break;                                \
     case 2: L(PR, 2, false); break;                                \
+    case 4: if (sh) L(PR, 4, true); else L(PR, 4, false); break;   \
     default: if (sh) L(PR, 3, true); else L(PR, 3, false); break;  \
   }
   if (mixed) { M(float) } else { M(double) }
