@@ -74,12 +74,29 @@ void free_fft_tables(void *p) {
 
 // Lines per block of the z kernels, measured at nzd = 768: FP32 four (more
 // blocks per CU than eight, the XCD order keeps the tiles that share a line in
-// one L2); FP64 eight for the inverse transform, two for the forward one.
+// one L2); FP64 two in both directions, with the twiddles read through the
+// cache instead of copied to LDS (an FP64 table is 12 KB per block, which
+// would cut the blocks per CU; FP32 keeps its LDS copy, measured faster).
+#ifndef HST_ZG64_FWD
+#define HST_ZG64_FWD 2
+#endif
+#ifndef HST_ZG64_INV
+#define HST_ZG64_INV 2
+#endif
+#ifndef HST_ZTW_LDS64
+#define HST_ZTW_LDS64 0
+#endif
+#ifndef HST_ZTW_LDS32
+#define HST_ZTW_LDS32 1
+#endif
+template <typename R>
+constexpr bool ztw_lds() { return sizeof(R) == 4 ? HST_ZTW_LDS32 : HST_ZTW_LDS64; }
+
 template <typename R, int N, bool FWD>
 struct ZCfg {
   static constexpr int T = fft_threads(N);
   static constexpr int LDS_MAX = 100 * 1024;
-  static constexpr int G0 = sizeof(R) == 8 ? (FWD ? 2 : 8) : 4;
+  static constexpr int G0 = sizeof(R) == 8 ? (FWD ? HST_ZG64_FWD : HST_ZG64_INV) : 4;
   // LDS line stride: lanes of the transposed accesses walk the lines (stride LS)
   // and the points (stride 1); LS = 16 dwords mod 64 spreads them over all banks
   static constexpr int DW = (int)sizeof(Cx<R>) / 4, PER = 64 / DW;
@@ -125,8 +142,10 @@ __global__ void __launch_bounds__((ZCfg<R, NZD, false>::NT))
            Cx<R> *__restrict__ Hv, Dims D, int row0, int nrows, const Cx<R> *__restrict__ tw) {
   constexpr int G = ZCfg<R, NZD, false>::G, T = ZCfg<R, NZD, false>::T, LS = ZCfg<R, NZD, false>::LS;
   __shared__ Cx<R> lds[G * LS];
-  __shared__ Cx<R> twl[NZD];  // the twiddles: lanes of a stage read scattered entries
-  for (int k = threadIdx.x; k < NZD; k += blockDim.x) twl[k] = tw[k];
+  __shared__ Cx<R> twl[ztw_lds<R>() ? NZD : 1];  // the twiddles: lanes of a stage read scattered entries
+  if constexpr (ztw_lds<R>())
+    for (int k = threadIdx.x; k < NZD; k += blockDim.x) twl[k] = tw[k];
+  const Cx<R> *twp = ztw_lds<R>() ? twl : tw;
   int tile, rowc, c;
   xcd_tile(tile, rowc, c);
   const int row = row0 + rowc, ix0 = tile * G;
@@ -145,7 +164,7 @@ __global__ void __launch_bounds__((ZCfg<R, NZD, false>::NT))
   }
   __syncthreads();
   const int g = threadIdx.x / T, t = threadIdx.x % T;
-  fft_line<R, NZD, T, +1>(lds + g * LS, t, twl);
+  fft_line<R, NZD, T, +1>(lds + g * LS, t, twp);
   Cx<R> *dst = Hv + ((size_t)c * nrows + rowc) * NZD * D.NXp;
   for (int e = threadIdx.x; e < G * NZD; e += blockDim.x) {
     const int j = e % G, zp = e / G;
@@ -161,8 +180,10 @@ __global__ void __launch_bounds__((ZCfg<R, NZD, true>::NT))
            const Cx<R> *__restrict__ tw) {
   constexpr int G = ZCfg<R, NZD, true>::G, T = ZCfg<R, NZD, true>::T, LS = ZCfg<R, NZD, true>::LS;
   __shared__ Cx<R> lds[G * LS];
-  __shared__ Cx<R> twl[NZD];
-  for (int k = threadIdx.x; k < NZD; k += blockDim.x) twl[k] = tw[k];
+  __shared__ Cx<R> twl[ztw_lds<R>() ? NZD : 1];
+  if constexpr (ztw_lds<R>())
+    for (int k = threadIdx.x; k < NZD; k += blockDim.x) twl[k] = tw[k];
+  const Cx<R> *twp = ztw_lds<R>() ? twl : tw;
   int tile, rowc, p;
   xcd_tile(tile, rowc, p);
   const int row = row0 + rowc, ix0 = tile * G;
@@ -173,7 +194,7 @@ __global__ void __launch_bounds__((ZCfg<R, NZD, true>::NT))
   }
   __syncthreads();
   const int g = threadIdx.x / T, t = threadIdx.x % T;
-  fft_line<R, NZD, T, -1>(lds + g * LS, t, twl);
+  fft_line<R, NZD, T, -1>(lds + g * LS, t, twp);
   Cx<R> *dst = P0 + (size_t)p * pstride + (size_t)row * D.ncol + (size_t)ix0 * D.NZm;
   const int nz = D.nz;
   for (int j = 0; j < G; j++) {
