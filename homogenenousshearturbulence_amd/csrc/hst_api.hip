@@ -415,7 +415,8 @@ static int exact_shift_split(double dt_sub, double time) {
   if (rc) return rc;
   hipStream_t st = S.prof ? S.stream : S.side;
   if (!S.prof) HIPCHK(hipStreamWaitEvent(S.side, S.ev_fork, 0));
-  rc = run_on(st, "exact_shift_old", 2.0 * (2.0 * os + 2.0 * 16.0) / 16.0 * bytes_field(), [&] {
+  // an FP32 array keeps its eliminated rows in FP32 too: q in, XR out and in, q out
+  rc = run_on(st, "exact_shift_old", 2.0 * (4.0 * os) / 16.0 * bytes_field(), [&] {
     return launch_exact_shift(S.D, 2, qo, fo, S.XRo, S.d_f0, S.fc, S.d_der, S.PX, S.PZ, sx0, sz0, sx1, sz1, st);
   });
   if (rc) return rc;

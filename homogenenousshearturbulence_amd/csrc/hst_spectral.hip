@@ -614,10 +614,10 @@ int launch_penta(const Dims &D, int nsys, const int *kinds, double lambda, const
 // Forward sweep of D0 x = r for one lane, XR stored; r = src (RHS 0) or the
 // D1 stencil of src (RHS 1, ghost rows filled).  Also returns the border
 // rows' right-hand sides r(ny-2), r(ny-1).
-template <int RHS, typename ST = double2>
+template <int RHS, typename ST = double2, typename XT = double2>
 __device__ __forceinline__ void d0_forward(const Dims &D, const double *__restrict__ f0,
                                            const double *__restrict__ der, const ST *__restrict__ src,
-                                           double2 *__restrict__ XR, int col, C &x1, C &x2, C &sx0, C &sx1, C &r25,
+                                           XT *__restrict__ XR, int col, C &x1, C &x2, C &sx0, C &sx1, C &r25,
                                            C &r27) {
   const size_t nc = D.ncol;
   const int m = D.ny, mi = m - 2;
@@ -808,9 +808,11 @@ int launch_shift_phases(const Dims &D, const double *y, double dgx, double dgz, 
   return hipGetLastError();
 }
 
+// XR holds the eliminated rows in the precision of the array (an FP32 array's
+// D0 solution is kept in FP32: it carries no more digits than the array).
 template <typename QT>
 __device__ __forceinline__ void shift_column(const Dims &D, const D0Const &fc, QT *__restrict__ q,
-                                             double2 *__restrict__ XR, int col, const double *__restrict__ f0,
+                                             QT *__restrict__ XR, int col, const double *__restrict__ f0,
                                              const double *__restrict__ der, const double2 *__restrict__ PX,
                                              const double2 *__restrict__ PZ, double sx0s, double sz0s, double sx1s,
                                              double sz1s) {
@@ -819,7 +821,7 @@ __device__ __forceinline__ void shift_column(const Dims &D, const D0Const &fc, Q
   const size_t nc = D.ncol;
   const int m = D.ny;
   C x1, x2, sx0, sx1, r25, r27;
-  d0_forward<0, QT>(D, f0, der, q, XR, col, x1, x2, sx0, sx1, r25, r27);
+  d0_forward<0, QT, QT>(D, f0, der, q, XR, col, x1, x2, sx0, sx1, r25, r27);
   const C ph0 = cexpi(-(D.alfa0 * ix * sx0s + D.beta0 * iz * sz0s));
   C b1, b2;
   d0_border(fc, ph0, x1, x2, sx0, sx1, r25, r27, b1, b2);
@@ -878,7 +880,8 @@ __global__ void __launch_bounds__(256)
   if (col >= D.ncol) return;
   const int k = blockIdx.y;
   if (ql.f32[k])
-    shift_column<float2>(D, fc, (float2 *)ql.q[k], ql.xr[k], col, f0, der, PX, PZ, sx0s, sz0s, sx1s, sz1s);
+    shift_column<float2>(D, fc, (float2 *)ql.q[k], (float2 *)ql.xr[k], col, f0, der, PX, PZ, sx0s, sz0s, sx1s,
+                         sz1s);
   else
     shift_column<double2>(D, fc, (double2 *)ql.q[k], ql.xr[k], col, f0, der, PX, PZ, sx0s, sz0s, sx1s, sz1s);
 }
