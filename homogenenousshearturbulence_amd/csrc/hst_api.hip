@@ -53,6 +53,10 @@ struct State {
   D0Factor f0;
   D0Const fc;
   double *d_der = nullptr, *d_y = nullptr, *d_dyl = nullptr, *d_f0 = nullptr, *d_part = nullptr;
+  // the D0 factorisation of the reversed line (the exact shift fused with the solves)
+  D0Factor f0r;
+  D0Const fcr;
+  double *d_f0r = nullptr;
   size_t part_cap = 0;
   double2 *F[9] = {nullptr};
   void *P = nullptr, *Hv = nullptr, *Hp = nullptr, *ws = nullptr;
@@ -394,7 +398,8 @@ static int exact_shift(double dt_sub, double time) {
 // it next), oldrhs on the side stream, joined before the next buildrhs.  With
 // timing on both halves run on the main stream, so that every kernel's time
 // is its own.
-static int exact_shift_split(double dt_sub, double time) {
+// fused: the rhs half is left to the fused shift-and-solve of the time step
+static int exact_shift_split(double dt_sub, double time, bool fused = false) {
   const double dgx = S.deck.S * dt_sub, dgz = s2_integral(time, time + dt_sub);
   double sx0, sz0, sx1, sz1;
   shear_shifts(time, sx0, sz0);
@@ -408,11 +413,14 @@ static int exact_shift_split(double dt_sub, double time) {
       }))
     return rc;
   HIPCHK(hipEventRecord(S.ev_fork, S.stream));
-  int rc = run("exact_shift", 2.0 * (2.0 * 16.0 + 2.0 * 16.0) / 16.0 * bytes_field(), [&] {
-    return launch_exact_shift(S.D, 2, qr, fr, xr, S.d_f0, S.fc, S.d_der, S.PX, S.PZ, sx0, sz0, sx1, sz1,
-                              S.stream);
-  });
-  if (rc) return rc;
+  int rc = 0;
+  if (!fused) {
+    rc = run("exact_shift", 2.0 * (2.0 * 16.0 + 2.0 * 16.0) / 16.0 * bytes_field(), [&] {
+      return launch_exact_shift(S.D, 2, qr, fr, xr, S.d_f0, S.fc, S.d_der, S.PX, S.PZ, sx0, sz0, sx1, sz1,
+                                S.stream);
+    });
+    if (rc) return rc;
+  }
   hipStream_t st = S.prof ? S.stream : S.side;
   if (!S.prof) HIPCHK(hipStreamWaitEvent(S.side, S.ev_fork, 0));
   // an FP32 array keeps its eliminated rows in FP32 too: q in, XR out and in, q out
@@ -498,6 +506,32 @@ static int linsolve(double lambda, double time) {
   return ghosts(2, g13, sx, sz);
 }
 
+// The exact shift of rhs1, rhs2 from t0 to t1 and linsolve at t1 with the
+// factor tables tabs: one fused pass for both (the shifted right-hand sides
+// stay in registers), then the ghosts of v, dv/dy with u and w, their ghosts.
+static int shift_and_linsolve(double lambda, double t0, double t1, const void *const *tabs) {
+  double sx0, sz0, sx1, sz1;
+  shear_shifts(t0, sx0, sz0);
+  shear_shifts(t1, sx1, sz1);
+  double2 *xrs[2] = {S.XR[1], S.XR[0]}, *xrp[2] = {S.pws[0].XR, S.pws[1].XR};
+  const void *tb[2] = {tabs[0], tabs[1]};
+  // per system: q in, the shift's eliminated rows out and in, the solve's out and in, the solution out
+  int rc = run("shift_penta", 2.0 * 6.0 * bytes_field(), [&] {
+    return launch_shift_penta(S.D, S.F[HST_RHS2], S.F[HST_RHS1], xrs, xrp, tb, S.F[HST_V2], S.F[HST_V1], S.d_f0r,
+                              S.fcr, S.d_der, S.PX, S.PZ, sx0, sz0, sx1, sz1, lambda, S.stream);
+  });
+  if (rc) return rc;
+  double2 *g2[1] = {S.F[HST_V2]};
+  if ((rc = ghosts(1, g2, sx1, sz1))) return rc;
+  rc = run("dy_recover", 6.0 * bytes_field(), [&] {
+    return launch_d0solve(S.D, 1, 1, S.F[HST_V2], S.F[HST_V1], S.F[HST_V3], S.XR[0], S.d_f0, S.fc, S.d_der, S.d_dyl,
+                          sx1, sz1, nullptr, nullptr, S.stream);
+  });
+  if (rc) return rc;
+  double2 *g13[2] = {S.F[HST_V1], S.F[HST_V3]};
+  return ghosts(2, g13, sx1, sz1);
+}
+
 // the main stream waits for the oldrhs shift of the side stream
 static int join_side() {
   if (!S.side_pending) return 0;
@@ -532,7 +566,7 @@ static void release() {
   if (S.stream) (void)hipStreamSynchronize(S.stream);
   if (S.side) (void)hipStreamSynchronize(S.side);
   for (int i = 1; i <= 8; i++) if (S.F[i]) (void)hipFree(S.F[i]);
-  void *ptrs[] = {S.P, S.Hv, S.Hp, S.ws, S.d_der, S.d_y, S.d_dyl, S.d_f0, S.d_part};
+  void *ptrs[] = {S.P, S.Hv, S.Hp, S.ws, S.d_der, S.d_y, S.d_dyl, S.d_f0, S.d_part, S.d_f0r};
   for (void *p : ptrs) if (p) (void)hipFree(p);
   for (auto &t : S.ptab)
     for (void *p : t) if (p) (void)hipFree(p);
@@ -594,6 +628,17 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
   Dims &D = S.D;
   d0_factor(S.der, D.ny, S.f0);
   S.fc = d0_const(S.f0);
+  {
+    // the reversed line: row i of it is row ny-1-i, its stencils mirrored
+    std::vector<double> dr(S.der.size());
+    const int ny = D.ny;
+    for (int n = 0; n < 4; n++)
+      for (int j = -2; j <= 2; j++)
+        for (int i = 0; i < ny; i++)
+          dr[(size_t)i + (size_t)ny * (n + 4 * (j + 2))] = S.der[(size_t)(ny - 1 - i) + (size_t)ny * (n + 4 * (-j + 2))];
+    d0_factor(dr, ny, S.f0r);
+    S.fcr = d0_const(S.f0r);
+  }
 
   HIPCHK(hipStreamCreate(&S.stream));
   HIPCHK(hipStreamCreate(&S.side));
@@ -676,6 +721,8 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
   HIPCHK(hipMemcpy(S.d_dyl, S.dyl.data(), S.dyl.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&S.d_f0, S.f0.rows.size() * sizeof(double)));
   HIPCHK(hipMemcpy(S.d_f0, S.f0.rows.data(), S.f0.rows.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&S.d_f0r, S.f0r.rows.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(S.d_f0r, S.f0r.rows.data(), S.f0r.rows.size() * sizeof(double), hipMemcpyHostToDevice));
   S.part_cap = std::max<size_t>((size_t)D.NYg * D.nzd, (size_t)((D.ncol + 255) / 256) * NSTAT + 64);
   HIPCHK(hipMalloc(&S.d_part, S.part_cap * sizeof(double)));
   if (make_fft_tables(D, S.mixed, &S.tables)) return fail("twiddle tables");
@@ -868,9 +915,15 @@ int hst_gpu_timestep(double deltat, double *time, double *cfl) {
     if ((rc = damping_update(t))) return rc;
     rc = buildrhs(ode, deltat, t, S.deck.linear ? 0 : (six ? 3 : 4), fuse_shift, dgx, dgz, true);
     if (rc) return rc;
-    if (S.deck.exact_shift && (rc = exact_shift_split(dt_sub, t))) return rc;
+    // with the factor tables the rhs half of the exact shift is fused with the solves
+    const double lam = ode[0] / deltat;
+    const void *const *tabs = S.deck.exact_shift ? penta_tables(lam) : nullptr;
+    const bool fused = tabs != nullptr;
+    if (S.deck.exact_shift && (rc = exact_shift_split(dt_sub, t, fused))) return rc;
+    const double t0 = t;
     t = t + dt_sub;
-    if ((rc = linsolve(ode[0] / deltat, t))) return rc;
+    rc = fused ? shift_and_linsolve(lam, t0, t, tabs) : linsolve(lam, t);
+    if (rc) return rc;
     if ((rc = stokes_apply(t))) return rc;
   }
   *time = t;

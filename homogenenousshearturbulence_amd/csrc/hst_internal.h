@@ -123,6 +123,16 @@ int launch_d0solve(const Dims &D, int rhs_mode, int out_mode, const double2 *src
 int launch_exact_shift(const Dims &D, int nq, void *const *q, const int *f32, double2 *const *XR,
                        const double *f0rows, const D0Const &fc, const double *der, const double2 *PX,
                        const double2 *PZ, double sx0, double sz0, double sx1, double sz1, hipStream_t s);
+// The exact shift of rhs2 and rhs1 and their D2V and ETA line solves at lambda
+// in one launch (v2, eta out; the shifted right-hand sides are not stored):
+// f0r, fcr the D0 factorisation of the reversed line (d0_factor of the
+// mirrored stencils); xrs, xrp the two sweeps' workspaces per system
+// (distinct); tabs the D2V and ETA factor tables.
+int launch_shift_penta(const Dims &D, const double2 *q2, const double2 *q1, double2 *const xrs[2],
+                       double2 *const xrp[2], const void *const tabs[2], double2 *v2, double2 *eta,
+                       const double *f0r, const D0Const &fcr, const double *der, const double2 *PX,
+                       const double2 *PZ, double sx0, double sz0, double sx1, double sz1, double lambda,
+                       hipStream_t s);
 // PX[i][ix] = exp(-i alfa0 ix dgx y_i) (ny x (nx+1)), PZ[i][iz+nz] = exp(-i beta0 iz dgz y_i) (ny x (2nz+1)).
 int launch_shift_phases(const Dims &D, const double *y, double dgx, double dgz, double2 *PX, double2 *PZ,
                         hipStream_t s);

@@ -896,6 +896,197 @@ int launch_exact_shift(const Dims &D, int nq, void *const *q, const int *f32, do
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ exact shift + line solve
+
+// The exact shift of one right-hand side (shift_unweighted,
+// src/hst_equations.f90:263-293) and its line solve (line_solve,
+// src/hst_linsolve.f90:282-483) in one pass per column, with the shifted
+// right-hand side handed from one to the other in registers.  The D0 solve
+// of the shift runs over the rows in reverse order (the reversed line is the
+// same cyclic system with the stencils mirrored and the wrap phase
+// conjugated: f0r, fcr), so that its back substitution yields x in ascending
+// rows; the re-weighted rows q = D0 x' (x' = x times the node phase, ghosts at
+// the new shifts) then come out ascending, the order the line solve's
+// elimination takes them.  Rows 0, 1 of x come from the reversed border
+// system, rows ny-2, ny-1 first from the record (for the ghost rows below row
+// 0), then from the back substitution.
+#ifndef HST_SP_PF
+#define HST_SP_PF 4
+#endif
+template <int K>
+__device__ __forceinline__ void shift_penta_line(const Dims &D, int col, const D0Const &fcr,
+                                                 const double *__restrict__ f0, const double *__restrict__ der,
+                                                 const double2 *__restrict__ q, double2 *__restrict__ XRs,
+                                                 const double2 *__restrict__ PX, const double2 *__restrict__ PZ,
+                                                 double sx0s, double sz0s, double sx1s, double sz1s, double lam,
+                                                 double2 *__restrict__ XRp, const double4 *__restrict__ tab,
+                                                 double2 *__restrict__ dst) {
+  int ix, iz;
+  col_ix_iz(D, col, ix, iz);
+  const size_t nc = D.ncol;
+  const int m = D.ny, mi = m - 2;
+  if (ix == 0 && iz == 0 && K == KD2V) {  // the (0,0) D2V line is zero (src/hst_linsolve.f90:199,221-224)
+    for (int i = 0; i < m; i++) st(dst + (size_t)(i + 2) * nc + col, C{0.0, 0.0});
+    return;
+  }
+  // 1. the reversed D0 elimination: reversed row k is row m-1-k
+  C x1{0, 0}, x2{0, 0}, sx0{0, 0}, sx1{0, 0};
+  rows_ascending<HST_SP_PF, C>(
+      mi, [&](int k) { return ld(q + (size_t)(m - 1 - k + 2) * nc + col); },
+      [&](int k, const C &v) {
+        C bx = v;
+        if (k >= 2) bx = bx - F0(k, 0) * x2;
+        if (k >= 1) bx = bx - F0(k, 1) * x1;
+        bx = bx * F0(k, 2);
+        st(XRs + (size_t)k * nc + col, bx);
+        sx0 = sx0 + F0(k, 7) * bx;
+        sx1 = sx1 + F0(k, 8) * bx;
+        x2 = x1;
+        x1 = bx;
+      });
+  const C r25 = ld(q + (size_t)(1 + 2) * nc + col), r27 = ld(q + (size_t)(0 + 2) * nc + col);
+  const C ph0 = cexpi(-(D.alfa0 * ix * sx0s + D.beta0 * iz * sz0s));
+  C b1, b2;  // reversed rows m-2, m-1: x(1), x(0)
+  d0_border(fcr, conj(ph0), x1, x2, sx0, sx1, r25, r27, b1, b2);
+  const C yb1 = ph0 * b1, yb2 = ph0 * b2;
+  const double *rr = fcr.rec;
+  const C xr0 = sx0 - ph0 * (rr[3] * b1 + rr[4] * b2) - (rr[5] * b1 + rr[6] * b2);    // x(m-1)
+  const C xr1 = sx1 - ph0 * (rr[9] * b1 + rr[10] * b2) - (rr[11] * b1 + rr[12] * b2);  // x(m-2)
+  auto phase = [&](int i) { return ld(PX + (size_t)i * D.NXm + ix) * ld(PZ + (size_t)i * D.NZm + iz + D.nz); };
+  const C ph1 = cexpi(-(D.alfa0 * ix * sx1s + D.beta0 * iz * sz1s));
+  const C xp0 = b2 * phase(0), xp1 = b1 * phase(1);
+  // 2. the line solve's elimination, fed row by row
+  const double al = D.alfa0 * ix, be = D.beta0 * iz;
+  const double kk = al * al + be * be, ni = D.ni;
+  PentaFactor f;
+  C px1{0, 0}, px2{0, 0}, psx0{0, 0}, psx1{0, 0}, r25p{0, 0}, r27p{0, 0};
+  auto feed = [&](int i, const C &bin) {
+    if (i < mi) {
+      double l2, l1, rp, t1, t2, p, qq;
+      f.step<K>(i, lam, ni, kk, der, l2, l1, rp, t1, t2, p, qq);
+      C bx = bin;
+      if (i >= 2) bx = bx - l2 * px2;
+      if (i >= 1) bx = bx - l1 * px1;
+      bx = bx * rp;
+      st(XRp + (size_t)i * nc + col, bx);
+      psx0 = psx0 + f.ww * bx;
+      psx1 = psx1 + f.vv * bx;
+      px2 = px1;
+      px1 = bx;
+    } else if (i == mi) {
+      r25p = bin;
+    } else {
+      r27p = bin;
+    }
+  };
+  // window W = x'(i-2 .. i+1) for the row i about to be emitted; below row 0
+  // the ghosts x'(-k) = x'(m-k) conj(ph1)
+  C W0 = xr1 * phase(m - 2) * conj(ph1), W1 = xr0 * phase(m - 1) * conj(ph1), W2 = xp0, W3 = xp1;
+  auto emit = [&](int i, const C &xnext) {  // xnext = x'(i + 2)
+    C acc{0, 0};
+    acc = acc + DER(der, i, 0, -2) * W0;
+    acc = acc + DER(der, i, 0, -1) * W1;
+    acc = acc + DER(der, i, 0, 0) * W2;
+    acc = acc + DER(der, i, 0, 1) * W3;
+    acc = acc + DER(der, i, 0, 2) * xnext;
+    W0 = W1; W1 = W2; W2 = W3; W3 = xnext;
+    feed(i, acc);
+  };
+  struct Row {
+    C xr, ph;
+  };
+  C xk1 = b1, xk2 = b2;
+  rows_descending<PF_BWD, Row>(
+      mi, [&](int k) { return Row{ld(XRs + (size_t)k * nc + col), phase(m - 1 - k)}; },
+      [&](int k, const Row &r) {
+        const C xk = r.xr - F0(k, 5) * yb1 - F0(k, 6) * yb2 - F0(k, 3) * xk1 - F0(k, 4) * xk2;
+        xk2 = xk1;
+        xk1 = xk;
+        emit(m - 1 - k - 2, xk * r.ph);  // x'(m-1-k) completes row m-3-k
+      });
+  emit(m - 2, xp0 * ph1);  // x'(m) = x'(0) ph1
+  emit(m - 1, xp1 * ph1);  // x'(m+1) = x'(1) ph1
+  // 3. the line solve's border system and back substitution (as penta_line)
+  double rq[25];
+  rq[3] = f.sp0; rq[4] = f.sq0; rq[5] = f.w2 * f.u22 + f.w1 * f.u11; rq[6] = f.w1 * f.u12;
+  rq[9] = f.sp1; rq[10] = f.sq1; rq[11] = f.v2 * f.u22 + f.v1 * f.u11; rq[12] = f.v1 * f.u12;
+  rq[15] = f.p2 - f.u21 * f.p1; rq[16] = f.q2 - f.u21 * f.q1; rq[17] = f.u22 - f.u21 * f.u11; rq[18] = -f.u21 * f.u12;
+  rq[21] = f.p1; rq[22] = f.q1; rq[23] = f.u11; rq[24] = f.u12;
+  const C xm2 = px2 - f.u21 * px1, xm1 = px1;
+  const int ga = m - 2;
+  C c1, c2;
+  border_solve(rq, coefk<K>(lam, ni, kk, der, ga, -2), coefk<K>(lam, ni, kk, der, ga, -1),
+               coefk<K>(lam, ni, kk, der, ga, 0), coefk<K>(lam, ni, kk, der, ga, 1),
+               coefk<K>(lam, ni, kk, der, ga, 2), coefk<K>(lam, ni, kk, der, ga + 1, -2),
+               coefk<K>(lam, ni, kk, der, ga + 1, -1), coefk<K>(lam, ni, kk, der, ga + 1, 0),
+               coefk<K>(lam, ni, kk, der, ga + 1, 1), coefk<K>(lam, ni, kk, der, ga + 1, 2), ph1, psx0, psx1, xm2,
+               xm1, r25p, r27p, c1, c2);
+  st(dst + (size_t)(m - 2 + 2) * nc + col, c1);
+  st(dst + (size_t)(m - 1 + 2) * nc + col, c2);
+  const C cph1 = conj(ph1);
+  const C zb1 = cph1 * c1, zb2 = cph1 * c2;
+  C zk1 = c1, zk2 = c2;
+  const int ng = (D.nx + 1) * (D.nz + 1), g = penta_group(D, ix, iz);
+  rows_descending<PF_BWD, WSRow>(
+      mi,
+      [&](int i) {
+        const double4 t = tab[(size_t)i * ng + g];
+        return WSRow{ld(XRp + (size_t)i * nc + col), t.z, t.w, t.x, t.y};
+      },
+      [&](int i, const WSRow &w) {
+        const C zk = w.xr - w.b1 * zb1 - w.b2 * zb2 - w.u1 * zk1 - w.u2 * zk2;
+        st(dst + (size_t)(i + 2) * nc + col, zk);
+        zk2 = zk1;
+        zk1 = zk;
+      });
+}
+
+struct ShiftPentaArgs {
+  const double2 *q[2];
+  double2 *xrs[2], *xrp[2], *dst[2];
+  const double4 *tab[2];
+};
+
+// three wavefronts per SIMD (at most 168 VGPRs) and loads four rows ahead in
+// the reversed elimination: measured fastest (four wavefronts spill)
+#ifndef HST_SP_WAVES
+#define HST_SP_WAVES 3
+#endif
+// blockIdx.y: system 0 (D2V on rhs2) or 1 (ETA on rhs1)
+__global__ void __launch_bounds__(256, HST_SP_WAVES)
+    k_shift_penta(Dims D, ShiftPentaArgs a, D0Const fcr, const double *__restrict__ f0r,
+                  const double *__restrict__ der, const double2 *__restrict__ PX, const double2 *__restrict__ PZ,
+                  double sx0s, double sz0s, double sx1s, double sz1s, double lam) {
+  const int l = xcd_linear();
+  const int sys = l / gridDim.x;
+  const int col = (l % gridDim.x) * blockDim.x + threadIdx.x;
+  if (col >= D.ncol) return;
+  if (sys == 0)
+    shift_penta_line<KD2V>(D, col, fcr, f0r, der, a.q[0], a.xrs[0], PX, PZ, sx0s, sz0s, sx1s, sz1s, lam, a.xrp[0],
+                           a.tab[0], a.dst[0]);
+  else
+    shift_penta_line<KETA>(D, col, fcr, f0r, der, a.q[1], a.xrs[1], PX, PZ, sx0s, sz0s, sx1s, sz1s, lam, a.xrp[1],
+                           a.tab[1], a.dst[1]);
+}
+
+int launch_shift_penta(const Dims &D, const double2 *q2, const double2 *q1, double2 *const xrs[2],
+                       double2 *const xrp[2], const void *const tabs[2], double2 *v2, double2 *eta,
+                       const double *f0r, const D0Const &fcr, const double *der, const double2 *PX,
+                       const double2 *PZ, double sx0, double sz0, double sx1, double sz1, double lambda,
+                       hipStream_t s) {
+  ShiftPentaArgs a;
+  a.q[0] = q2; a.q[1] = q1;
+  a.dst[0] = v2; a.dst[1] = eta;
+  for (int k = 0; k < 2; k++) {
+    a.xrs[k] = xrs[k];
+    a.xrp[k] = xrp[k];
+    a.tab[k] = (const double4 *)tabs[k];
+  }
+  dim3 grid((D.ncol + 255) / 256, 2);
+  hipLaunchKernelGGL(k_shift_penta, grid, dim3(256), 0, s, D, a, fcr, f0r, der, PX, PZ, sx0, sz0, sx1, sz1, lambda);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ ghosts
 
 struct FList {
