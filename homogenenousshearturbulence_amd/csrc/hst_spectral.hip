@@ -911,7 +911,10 @@ int launch_exact_shift(const Dims &D, int nq, void *const *q, const int *f32, do
 // system, rows ny-2, ny-1 first from the record (for the ghost rows below row
 // 0), then from the back substitution.
 #ifndef HST_SP_PF
-#define HST_SP_PF 4
+#define HST_SP_PF 2
+#endif
+#ifndef HST_SP_PFB
+#define HST_SP_PFB 2
 #endif
 template <int K>
 __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, const D0Const &fcr,
@@ -996,7 +999,7 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, const D
     C xr, ph;
   };
   C xk1 = b1, xk2 = b2;
-  rows_descending<PF_BWD, Row>(
+  rows_descending<HST_SP_PFB, Row>(
       mi, [&](int k) { return Row{ld(XRs + (size_t)k * nc + col), phase(m - 1 - k)}; },
       [&](int k, const Row &r) {
         const C xk = r.xr - F0(k, 5) * yb1 - F0(k, 6) * yb2 - F0(k, 3) * xk1 - F0(k, 4) * xk2;
@@ -1027,7 +1030,7 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, const D
   const C zb1 = cph1 * c1, zb2 = cph1 * c2;
   C zk1 = c1, zk2 = c2;
   const int ng = (D.nx + 1) * (D.nz + 1), g = penta_group(D, ix, iz);
-  rows_descending<PF_BWD, WSRow>(
+  rows_descending<HST_SP_PFB, WSRow>(
       mi,
       [&](int i) {
         const double4 t = tab[(size_t)i * ng + g];
@@ -1047,8 +1050,9 @@ struct ShiftPentaArgs {
   const double4 *tab[2];
 };
 
-// three wavefronts per SIMD (at most 168 VGPRs) and loads four rows ahead in
-// the reversed elimination: measured fastest (four wavefronts spill)
+// three wavefronts per SIMD (at most 168 VGPRs) and loads two rows ahead in
+// each sweep: measured fastest (four wavefronts spill; deeper prefetch costs
+// the registers that the three sweeps' states need)
 #ifndef HST_SP_WAVES
 #define HST_SP_WAVES 3
 #endif
