@@ -400,7 +400,8 @@ struct WSRow {
   double b1, b2, u1, u2;
 };
 
-constexpr int PF_FWD = 8, PF_BWD = 4, PF_PENTA = 4;
+// rows of loads in flight per sweep (measured at nx = nz = 256, ny = 513)
+constexpr int PF_FWD = 8, PF_BWD = 2, PF_PENTA = 4;
 
 // The elimination of one cyclic pentadiagonal line with the rows generated
 // from der (line_solve, src/hst_linsolve.f90:282-380, one slab), split into
@@ -910,12 +911,10 @@ int launch_exact_shift(const Dims &D, int nq, void *const *q, const int *f32, do
 // elimination takes them.  Rows 0, 1 of x come from the reversed border
 // system, rows ny-2, ny-1 first from the record (for the ghost rows below row
 // 0), then from the back substitution.
-#ifndef HST_SP_PF
-#define HST_SP_PF 2
-#endif
-#ifndef HST_SP_PFB
-#define HST_SP_PFB 2
-#endif
+// three wavefronts per SIMD (at most 168 VGPRs) and loads two rows ahead in
+// each sweep: measured fastest (four wavefronts spill; deeper prefetch costs
+// the registers that the three sweeps' states need)
+constexpr int SP_WAVES = 3, SP_PF = 2;
 template <int K>
 __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, const D0Const &fcr,
                                                  const double *__restrict__ f0, const double *__restrict__ der,
@@ -934,7 +933,7 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, const D
   }
   // 1. the reversed D0 elimination: reversed row k is row m-1-k
   C x1{0, 0}, x2{0, 0}, sx0{0, 0}, sx1{0, 0};
-  rows_ascending<HST_SP_PF, C>(
+  rows_ascending<SP_PF, C>(
       mi, [&](int k) { return ld(q + (size_t)(m - 1 - k + 2) * nc + col); },
       [&](int k, const C &v) {
         C bx = v;
@@ -999,7 +998,7 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, const D
     C xr, ph;
   };
   C xk1 = b1, xk2 = b2;
-  rows_descending<HST_SP_PFB, Row>(
+  rows_descending<SP_PF, Row>(
       mi, [&](int k) { return Row{ld(XRs + (size_t)k * nc + col), phase(m - 1 - k)}; },
       [&](int k, const Row &r) {
         const C xk = r.xr - F0(k, 5) * yb1 - F0(k, 6) * yb2 - F0(k, 3) * xk1 - F0(k, 4) * xk2;
@@ -1030,7 +1029,7 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, const D
   const C zb1 = cph1 * c1, zb2 = cph1 * c2;
   C zk1 = c1, zk2 = c2;
   const int ng = (D.nx + 1) * (D.nz + 1), g = penta_group(D, ix, iz);
-  rows_descending<HST_SP_PFB, WSRow>(
+  rows_descending<SP_PF, WSRow>(
       mi,
       [&](int i) {
         const double4 t = tab[(size_t)i * ng + g];
@@ -1050,14 +1049,8 @@ struct ShiftPentaArgs {
   const double4 *tab[2];
 };
 
-// three wavefronts per SIMD (at most 168 VGPRs) and loads two rows ahead in
-// each sweep: measured fastest (four wavefronts spill; deeper prefetch costs
-// the registers that the three sweeps' states need)
-#ifndef HST_SP_WAVES
-#define HST_SP_WAVES 3
-#endif
 // blockIdx.y: system 0 (D2V on rhs2) or 1 (ETA on rhs1)
-__global__ void __launch_bounds__(256, HST_SP_WAVES)
+__global__ void __launch_bounds__(256, SP_WAVES)
     k_shift_penta(Dims D, ShiftPentaArgs a, D0Const fcr, const double *__restrict__ f0r,
                   const double *__restrict__ der, const double2 *__restrict__ PX, const double2 *__restrict__ PZ,
                   double sx0s, double sz0s, double sx1s, double sz1s, double lam) {
