@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -93,6 +94,7 @@ struct State {
   double extra[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // the last outstats' layer and damping lines
   int64_t bytes_held = 0;
   bool prof = false;
+  bool serial = false;  // the oldrhs shift on the main stream (HST_SERIAL=1)
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, ProfEntry> table;
@@ -421,14 +423,18 @@ static int exact_shift_split(double dt_sub, double time, bool fused = false) {
     });
     if (rc) return rc;
   }
-  hipStream_t st = S.prof ? S.stream : S.side;
-  if (!S.prof) HIPCHK(hipStreamWaitEvent(S.side, S.ev_fork, 0));
+  // serial (the kernel timing pass, or HST_SERIAL=1 in the environment for an
+  // external profiler): on the main stream after the rhs half, each kernel
+  // timed alone
+  const bool serial = S.prof || S.serial;
+  hipStream_t st = serial ? S.stream : S.side;
+  if (!serial) HIPCHK(hipStreamWaitEvent(S.side, S.ev_fork, 0));
   // an FP32 array keeps its eliminated rows in FP32 too: q in, XR out and in, q out
   rc = run_on(st, "exact_shift_old", 2.0 * (4.0 * os) / 16.0 * bytes_field(), [&] {
     return launch_exact_shift(S.D, 2, qo, fo, S.XRo, S.d_f0, S.fc, S.d_der, S.PX, S.PZ, sx0, sz0, sx1, sz1, st);
   });
   if (rc) return rc;
-  if (!S.prof) {
+  if (!serial) {
     HIPCHK(hipEventRecord(S.ev_join, S.side));
     S.side_pending = true;
   }
@@ -644,6 +650,7 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
   HIPCHK(hipStreamCreate(&S.side));
   HIPCHK(hipEventCreateWithFlags(&S.ev_fork, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&S.ev_join, hipEventDisableTiming));
+  if (const char *e = std::getenv("HST_SERIAL")) S.serial = std::atoi(e) != 0;
   const size_t nf = (size_t)D.NYg * D.ncol;
   const size_t cs = S.mixed ? sizeof(float2) : sizeof(double2);
   int64_t held = 0;

@@ -210,8 +210,11 @@ int hst_gpu_memory(int64_t *bytes, int *chunk_rows);
 /* ------------------------------------------------------------------ measurement */
 
 /* Per-kernel timing with HIP events on the library's stream (hst_timer's
- * phases, src/hst_timer.f90, at kernel granularity).  enable = 1 starts
- * recording (and clears the table), 0 stops. */
+ * phases, src/hst_timer.f90, at kernel granularity), every kernel timed alone:
+ * while recording, the oldrhs shift that a time step otherwise runs on a second
+ * stream beside the solves runs on the main stream (HST_SERIAL=1 in the
+ * environment at hst_gpu_init does the same for an external profiler).
+ * enable = 1 starts recording (and clears the table), 0 stops. */
 int hst_gpu_profile(int enable);
 /* Number of kernels in the table; name, launches, total milliseconds and the
  * algorithmic HBM bytes of all launches of entry i. */
