@@ -117,6 +117,25 @@ struct XCfg {
   static constexpr int NT = G * T;
 };
 
+// ------------------------------------------------------------------ transform buffers
+
+// Layout of one (component, row) plane of Hv / Hp (nzd z points x NXp x modes).
+// Hv in FP32 is tiled by HTILE x modes, each tile z-major ([ix / HTILE][zp]
+// [ix % HTILE]): a block of the z inverse transform (HTILE modes, every z
+// point) writes one contiguous range instead of HTILE-mode pieces of every z
+// row, and the x pass reads its line in HTILE-mode pieces from lines that
+// neighbouring blocks share through L2.  Hp, and both in FP64, are [zp][ix]
+// (tiling Hp as well made the x pass's stores slower than the z forward
+// transform's loads gained).  B: 0 for Hv, 1 for Hp.
+constexpr int HTILE = 4;
+template <typename R, int B>
+__device__ __forceinline__ size_t hoff(int NZD, int NXp, int zp, int ix) {
+  if constexpr (sizeof(R) == 4 && B == 0)
+    return (size_t)(ix / HTILE) * ((size_t)NZD * HTILE) + (size_t)zp * HTILE + ix % HTILE;
+  else
+    return (size_t)zp * NXp + ix;
+}
+
 // ------------------------------------------------------------------ XCD-aware block order
 
 // Workgroups are dealt to the 8 XCDs round-robin by their linear id, and each
@@ -168,7 +187,7 @@ __global__ void __launch_bounds__((ZCfg<R, NZD, false>::NT))
   Cx<R> *dst = Hv + ((size_t)c * nrows + rowc) * NZD * D.NXp;
   for (int e = threadIdx.x; e < G * NZD; e += blockDim.x) {
     const int j = e % G, zp = e / G;
-    if (ix0 + j < D.NXm) dst[(size_t)zp * D.NXp + ix0 + j] = lds[j * LS + zp];
+    if (ix0 + j < D.NXm) dst[hoff<R, 0>(NZD, D.NXp, zp, ix0 + j)] = lds[j * LS + zp];
   }
 }
 
@@ -190,7 +209,7 @@ __global__ void __launch_bounds__((ZCfg<R, NZD, true>::NT))
   const Cx<R> *src = Hp + ((size_t)p * nrows + rowc) * NZD * D.NXp;
   for (int e = threadIdx.x; e < G * NZD; e += blockDim.x) {
     const int j = e % G, zp = e / G;
-    lds[j * LS + zp] = (ix0 + j < D.NXm) ? src[(size_t)zp * D.NXp + ix0 + j] : Cx<R>{(R)0, (R)0};
+    lds[j * LS + zp] = (ix0 + j < D.NXm) ? src[hoff<R, 1>(NZD, D.NXp, zp, ix0 + j)] : Cx<R>{(R)0, (R)0};
   }
   __syncthreads();
   const int g = threadIdx.x / T, t = threadIdx.x % T;
@@ -244,8 +263,8 @@ __global__ void __launch_bounds__((XCfg<R, M>::NT))
   Cx<R> *wl = work + g * LS;
   // X -> Z of the half-length complex inverse transform, per component
   for (int c = 0; c < 3; c++) {
-    const Cx<R> *src = Hv + (((size_t)c * nrows + rowc) * NZD + zp) * D.NXp;
-    for (int k = t; k <= nx; k += T) wl[k] = src[k];
+    const Cx<R> *src = Hv + ((size_t)c * nrows + rowc) * NZD * D.NXp;
+    for (int k = t; k <= nx; k += T) wl[k] = src[hoff<R, 0>(NZD, D.NXp, zp, k)];
     __syncthreads();
     Cx<R> *vl = vel + (c * G + g) * LS;
     for (int k = t; k < M; k += T) {
@@ -303,12 +322,12 @@ __global__ void __launch_bounds__((XCfg<R, M>::NT))
     }
     __syncthreads();
     fft_line<R, M, T, -1>(wl, t, twx);
-    Cx<R> *dst = Hp + (((size_t)p * nrows + rowc) * NZD + zp) * D.NXp;
+    Cx<R> *dst = Hp + ((size_t)p * nrows + rowc) * NZD * D.NXp;
     for (int k = t; k <= nx; k += T) {
       const Cx<R> Z = wl[k], Zc = conj(wl[(M - k) % M]);
       const Cx<R> e = twh[k];  // exp(-i pi k / M)
       const Cx<R> s = (Z + Zc) * (R)0.5, d = (Z - Zc) * (R)0.5;
-      dst[k] = s - mul_i(e * d);
+      dst[hoff<R, 1>(NZD, D.NXp, zp, k)] = s - mul_i(e * d);
     }
     __syncthreads();
   }
@@ -389,16 +408,16 @@ __device__ __forceinline__ void xl_fft(Cx<R> (&v)[XL<M>::E], Cx<R> *L, int t, co
 // Z(k) = (X(k) + conj X(M-k)) + i exp(i pi k/M) (X(k) - conj X(M-k)) at k = t + T e:
 // the half-length complex input of the length-2M inverse real transform.
 template <typename R, int M>
-__device__ __forceinline__ void xl_c2r_load(Cx<R> (&dst)[XL<M>::E], const Cx<R> *__restrict__ src, bool ok, int t,
-                                            int nx, const Cx<R> *__restrict__ twh) {
+__device__ __forceinline__ void xl_c2r_load(Cx<R> (&dst)[XL<M>::E], const Cx<R> *__restrict__ src, int NZD, int NXp,
+                                            int zp, bool ok, int t, int nx, const Cx<R> *__restrict__ twh) {
   constexpr int T = XL<M>::T, E = XL<M>::E;
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int k = t + T * e;
     Cx<R> a{(R)0, (R)0}, b{(R)0, (R)0};
-    if (ok && k <= nx) a = src[k];
+    if (ok && k <= nx) a = src[hoff<R, 0>(NZD, NXp, zp, k)];
     if (k == 0) a.y = (R)0;  // c2r: the imaginary part of the kx = 0 bin is ignored
-    if (ok && k > 0 && M - k <= nx) b = conj(src[M - k]);
+    if (ok && k > 0 && M - k <= nx) b = conj(src[hoff<R, 0>(NZD, NXp, zp, M - k)]);
     const Cx<R> ep = conj(twh[k]);  // exp(+i pi k / M)
     dst[e] = (a + b) + mul_i(ep * (a - b));
   }
@@ -426,15 +445,18 @@ __global__ void __launch_bounds__(64, (xline_min_waves<R, M>()))
   __syncthreads();
   const int g = threadIdx.x / T, t = threadIdx.x % T;
   const int NZD = D.nzd, nx = D.nx;
-  const int line = blockIdx.x * LPB + g;
+  // tiled Hv (FP32): neighbouring z points share cache lines, so neighbouring
+  // blocks stay on one XCD (one L2)
+  const int line = (sizeof(R) == 4 ? xcd_linear() : (int)blockIdx.x) * LPB + g;
   const bool ok = line < NZD * nrows;
   const int rowc = ok ? line / NZD : 0, zp = ok ? line - rowc * NZD : 0;
   Cx<R> *L = lds + g * PL;
   Cx<R> u[E], v[E], w[E];
   // C2R pre-processing of u, v, w straight from the x modes in HBM
-  xl_c2r_load<R, M>(u, Hv + (((size_t)0 * nrows + rowc) * NZD + zp) * D.NXp, ok, t, nx, twh);
-  xl_c2r_load<R, M>(v, Hv + (((size_t)1 * nrows + rowc) * NZD + zp) * D.NXp, ok, t, nx, twh);
-  xl_c2r_load<R, M>(w, Hv + (((size_t)2 * nrows + rowc) * NZD + zp) * D.NXp, ok, t, nx, twh);
+  const size_t plane = (size_t)NZD * D.NXp;
+  xl_c2r_load<R, M>(u, Hv + ((size_t)0 * nrows + rowc) * plane, NZD, D.NXp, zp, ok, t, nx, twh);
+  xl_c2r_load<R, M>(v, Hv + ((size_t)1 * nrows + rowc) * plane, NZD, D.NXp, zp, ok, t, nx, twh);
+  xl_c2r_load<R, M>(w, Hv + ((size_t)2 * nrows + rowc) * plane, NZD, D.NXp, zp, ok, t, nx, twh);
   xl_fft<R, M, +1, 0, 1>(u, L, t, twl);
   xl_fft<R, M, +1, 0, 1>(v, L, t, twl);
   xl_fft<R, M, +1, 0, 1>(w, L, t, twl);
@@ -480,7 +502,7 @@ __global__ void __launch_bounds__(64, (xline_min_waves<R, M>()))
 #pragma unroll
     for (int e = 0; e < E; e++) L[padx(t + T * e)] = z[e];
     __syncthreads();
-    Cx<R> *dst = Hp + (((size_t)p * nrows + rowc) * NZD + zp) * D.NXp;
+    Cx<R> *dst = Hp + ((size_t)p * nrows + rowc) * plane;
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int k = t + T * e;
@@ -488,7 +510,7 @@ __global__ void __launch_bounds__(64, (xline_min_waves<R, M>()))
         const Cx<R> Z = z[e], Zc = conj(L[padx((M - k) & (M - 1))]);
         const Cx<R> ep = twh[k];  // exp(-i pi k / M)
         const Cx<R> sm = (Z + Zc) * (R)0.5, df = (Z - Zc) * (R)0.5;
-        dst[k] = sm - mul_i(ep * df);
+        dst[hoff<R, 1>(NZD, D.NXp, zp, k)] = sm - mul_i(ep * df);
       }
     }
   }
