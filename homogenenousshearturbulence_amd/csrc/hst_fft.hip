@@ -120,18 +120,20 @@ struct XCfg {
 // ------------------------------------------------------------------ transform buffers
 
 // Layout of one (component, row) plane of Hv / Hp (nzd z points x NXp x modes).
-// Hv in FP32 is tiled by HTILE x modes, each tile z-major ([ix / HTILE][zp]
-// [ix % HTILE]): a block of the z inverse transform (HTILE modes, every z
-// point) writes one contiguous range instead of HTILE-mode pieces of every z
-// row, and the x pass reads its line in HTILE-mode pieces from lines that
-// neighbouring blocks share through L2.  Hp, and both in FP64, are [zp][ix]
-// (tiling Hp as well made the x pass's stores slower than the z forward
-// transform's loads gained).  B: 0 for Hv, 1 for Hp.
-constexpr int HTILE = 4;
+// Hv is tiled by the z inverse transform's modes per block (4 in FP32, 2 in
+// FP64), each tile z-major ([ix / HT][zp][ix % HT]): a block of that transform
+// writes one contiguous range instead of HT-mode pieces of every z row, and
+// the x pass reads its line in HT-mode pieces from lines that neighbouring
+// blocks share through L2.  Hp is [zp][ix] (tiling it as well made the x
+// pass's stores slower than the z forward transform's loads gained).
+// B: 0 for Hv, 1 for Hp.
+template <typename R>
+constexpr int htile() { return sizeof(R) == 4 ? 4 : 2; }
 template <typename R, int B>
 __device__ __forceinline__ size_t hoff(int NZD, int NXp, int zp, int ix) {
-  if constexpr (sizeof(R) == 4 && B == 0)
-    return (size_t)(ix / HTILE) * ((size_t)NZD * HTILE) + (size_t)zp * HTILE + ix % HTILE;
+  constexpr int HT = htile<R>();
+  if constexpr (HT > 0 && B == 0)
+    return (size_t)(ix / HT) * ((size_t)NZD * HT) + (size_t)zp * HT + ix % HT;
   else
     return (size_t)zp * NXp + ix;
 }
@@ -447,7 +449,7 @@ __global__ void __launch_bounds__(64, (xline_min_waves<R, M>()))
   const int NZD = D.nzd, nx = D.nx;
   // tiled Hv (FP32): neighbouring z points share cache lines, so neighbouring
   // blocks stay on one XCD (one L2)
-  const int line = (sizeof(R) == 4 ? xcd_linear() : (int)blockIdx.x) * LPB + g;
+  const int line = (htile<R>() > 0 ? xcd_linear() : (int)blockIdx.x) * LPB + g;
   const bool ok = line < NZD * nrows;
   const int rowc = ok ? line / NZD : 0, zp = ok ? line - rowc * NZD : 0;
   Cx<R> *L = lds + g * PL;
