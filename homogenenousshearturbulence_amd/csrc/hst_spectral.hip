@@ -458,6 +458,27 @@ struct PentaFactor {
 
 __device__ __forceinline__ int penta_group(const Dims &D, int ix, int iz) { return ix * (D.nz + 1) + (iz < 0 ? -iz : iz); }
 
+// Lane order of the line kernels that read the factor tables: slot p of the
+// (ix, |iz|) groups (p = ix (nz+1) + |iz|, the table's own index) runs the
+// column (ix, |iz|) in lane p%32 of the first half of wavefront p/32 and the
+// column (ix, -|iz|) in the same lane of the second half, so that one load of
+// a table row (and of a z phase) serves both; the second-half lane of iz = 0
+// idles.  Each half-wavefront reads 32 neighbouring columns.  False for an
+// idle lane.
+__device__ __forceinline__ bool pair_lane(const Dims &D, int L, int &ix, int &iz, int &col, int &g) {
+  const int h = (L >> 5) & 1;
+  g = (L >> 6) * 32 + (L & 31);
+  if (g >= D.NXm * (D.nz + 1)) return false;
+  ix = g / (D.nz + 1);
+  const int k = g - ix * (D.nz + 1);
+  if (h && k == 0) return false;
+  iz = h ? -k : k;
+  col = ix * D.NZm + iz + D.nz;
+  return true;
+}
+
+static int pair_lanes(const Dims &D) { return (D.NXm * (D.nz + 1) + 31) / 32 * 64; }
+
 // The matrix part of every (ix, |iz|) group for one kind and lambda:
 // tab[i][g] = (t1, t2, p, q) of row i.  Columns (ix, iz) and (ix, -iz) share
 // k^2 and so every bit of these rows.
@@ -483,12 +504,10 @@ __global__ void __launch_bounds__(256) k_penta_factor(Dims D, double lam, const 
 // der, record in registers, 2x2 border system, back substitution.  With a
 // factor table the workspace holds only the eliminated right-hand side.
 template <int K, bool TAB>
-__device__ __forceinline__ void penta_line(const Dims &D, int col, double lam, const double *__restrict__ der,
-                                           const double2 *__restrict__ src, double2 *__restrict__ dst,
-                                           const PentaWS &ws, const double4 *__restrict__ tab, double sx,
-                                           double sz) {
-  int ix, iz;
-  col_ix_iz(D, col, ix, iz);
+__device__ __forceinline__ void penta_line(const Dims &D, int col, int ix, int iz, int g, double lam,
+                                           const double *__restrict__ der, const double2 *__restrict__ src,
+                                           double2 *__restrict__ dst, const PentaWS &ws,
+                                           const double4 *__restrict__ tab, double sx, double sz) {
   const size_t nc = D.ncol;
   const int m = D.ny, mi = m - 2;
   if (ix == 0 && iz == 0 && K == KD2V) {
@@ -538,7 +557,7 @@ __device__ __forceinline__ void penta_line(const Dims &D, int col, double lam, c
   const C cph = conj(ph);
   const C yb1 = cph * b1, yb2 = cph * b2;
   C xk1 = b1, xk2 = b2;
-  const int ng = (D.nx + 1) * (D.nz + 1), g = penta_group(D, ix, iz);
+  const int ng = (D.nx + 1) * (D.nz + 1);
   rows_descending<PF_BWD, WSRow>(
       mi,
       [&](int i) {
@@ -562,14 +581,22 @@ __device__ __forceinline__ void penta_line(const Dims &D, int col, double lam, c
 template <int K0, int K1, bool TAB>
 __global__ void __launch_bounds__(256)
     k_penta(Dims D, PentaArgs pa, double lam, const double *__restrict__ der, double sx, double sz) {
-  // the +iz and -iz columns of an ix lie in neighbouring blocks and read the
-  // same factor rows: keep neighbouring blocks on one XCD (one L2)
+  // with the factor tables the +iz and -iz columns share a wavefront (one
+  // load of a table row serves both) and neighbouring blocks stay on one XCD
   const int l = TAB ? xcd_linear() : blockIdx.x + gridDim.x * blockIdx.y;
   const int sys = l / gridDim.x;
-  const int col = (l % gridDim.x) * blockDim.x + threadIdx.x;
-  if (col >= D.ncol) return;
-  if (sys == 0) penta_line<K0, TAB>(D, col, lam, der, pa.src[0], pa.dst[0], pa.ws[0], pa.tab[0], sx, sz);
-  else penta_line<K1, TAB>(D, col, lam, der, pa.src[1], pa.dst[1], pa.ws[1], pa.tab[1], sx, sz);
+  const int L = (l % gridDim.x) * blockDim.x + threadIdx.x;
+  int ix, iz, col, g;
+  if constexpr (TAB) {
+    if (!pair_lane(D, L, ix, iz, col, g)) return;
+  } else {
+    if (L >= D.ncol) return;
+    col = L;
+    col_ix_iz(D, col, ix, iz);
+    g = penta_group(D, ix, iz);
+  }
+  if (sys == 0) penta_line<K0, TAB>(D, col, ix, iz, g, lam, der, pa.src[0], pa.dst[0], pa.ws[0], pa.tab[0], sx, sz);
+  else penta_line<K1, TAB>(D, col, ix, iz, g, lam, der, pa.src[1], pa.dst[1], pa.ws[1], pa.tab[1], sx, sz);
 }
 
 size_t penta_table_bytes(const Dims &D) { return (size_t)(D.ny - 2) * (D.nx + 1) * (D.nz + 1) * sizeof(double4); }
@@ -595,7 +622,7 @@ int launch_penta(const Dims &D, int nsys, const int *kinds, double lambda, const
     pa.tab[k] = tabs ? (const double4 *)tabs[k] : nullptr;
     if (!pa.tab[k]) tab = false;
   }
-  dim3 grid((D.ncol + 255) / 256, nsys);
+  dim3 grid(((tab ? pair_lanes(D) : D.ncol) + 255) / 256, nsys);
 #define L(A, B, T) hipLaunchKernelGGL((k_penta<A, B, T>), grid, dim3(256), 0, s, D, pa, lambda, der, sx, sz)
   if (nsys == 2 && kinds[0] == KD2V && kinds[1] == KETA) {
     if (tab) L(KD2V, KETA, true); else L(KD2V, KETA, false);
@@ -916,15 +943,14 @@ int launch_exact_shift(const Dims &D, int nq, void *const *q, const int *f32, do
 // the registers that the three sweeps' states need)
 constexpr int SP_WAVES = 3, SP_PF = 2;
 template <int K>
-__device__ __forceinline__ void shift_penta_line(const Dims &D, int col, const D0Const &fcr,
+__device__ __forceinline__ void shift_penta_line(const Dims &D, int col, int ix, int iz, int g,
+                                                 const D0Const &fcr,
                                                  const double *__restrict__ f0, const double *__restrict__ der,
                                                  const double2 *__restrict__ q, double2 *__restrict__ XRs,
                                                  const double2 *__restrict__ PX, const double2 *__restrict__ PZ,
                                                  double sx0s, double sz0s, double sx1s, double sz1s, double lam,
                                                  double2 *__restrict__ XRp, const double4 *__restrict__ tab,
                                                  double2 *__restrict__ dst) {
-  int ix, iz;
-  col_ix_iz(D, col, ix, iz);
   const size_t nc = D.ncol;
   const int m = D.ny, mi = m - 2;
   if (ix == 0 && iz == 0 && K == KD2V) {  // the (0,0) D2V line is zero (src/hst_linsolve.f90:199,221-224)
@@ -954,7 +980,13 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, const D
   const double *rr = fcr.rec;
   const C xr0 = sx0 - ph0 * (rr[3] * b1 + rr[4] * b2) - (rr[5] * b1 + rr[6] * b2);    // x(m-1)
   const C xr1 = sx1 - ph0 * (rr[9] * b1 + rr[10] * b2) - (rr[11] * b1 + rr[12] * b2);  // x(m-2)
-  auto phase = [&](int i) { return ld(PX + (size_t)i * D.NXm + ix) * ld(PZ + (size_t)i * D.NZm + iz + D.nz); };
+  // the z phase of -k is the conjugate of that of k: both halves of a
+  // wavefront load the same row of PZ
+  const int az = iz < 0 ? -iz : iz;
+  auto phase = [&](int i) {
+    const C pz = ld(PZ + (size_t)i * D.NZm + az + D.nz);
+    return ld(PX + (size_t)i * D.NXm + ix) * (iz < 0 ? conj(pz) : pz);
+  };
   const C ph1 = cexpi(-(D.alfa0 * ix * sx1s + D.beta0 * iz * sz1s));
   const C xp0 = b2 * phase(0), xp1 = b1 * phase(1);
   // 2. the line solve's elimination, fed row by row
@@ -1028,7 +1060,7 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, const D
   const C cph1 = conj(ph1);
   const C zb1 = cph1 * c1, zb2 = cph1 * c2;
   C zk1 = c1, zk2 = c2;
-  const int ng = (D.nx + 1) * (D.nz + 1), g = penta_group(D, ix, iz);
+  const int ng = (D.nx + 1) * (D.nz + 1);
   rows_descending<SP_PF, WSRow>(
       mi,
       [&](int i) {
@@ -1056,13 +1088,13 @@ __global__ void __launch_bounds__(256, SP_WAVES)
                   double sx0s, double sz0s, double sx1s, double sz1s, double lam) {
   const int l = xcd_linear();
   const int sys = l / gridDim.x;
-  const int col = (l % gridDim.x) * blockDim.x + threadIdx.x;
-  if (col >= D.ncol) return;
+  int ix, iz, col, g;
+  if (!pair_lane(D, (l % gridDim.x) * blockDim.x + threadIdx.x, ix, iz, col, g)) return;
   if (sys == 0)
-    shift_penta_line<KD2V>(D, col, fcr, f0r, der, a.q[0], a.xrs[0], PX, PZ, sx0s, sz0s, sx1s, sz1s, lam, a.xrp[0],
+    shift_penta_line<KD2V>(D, col, ix, iz, g, fcr, f0r, der, a.q[0], a.xrs[0], PX, PZ, sx0s, sz0s, sx1s, sz1s, lam, a.xrp[0],
                            a.tab[0], a.dst[0]);
   else
-    shift_penta_line<KETA>(D, col, fcr, f0r, der, a.q[1], a.xrs[1], PX, PZ, sx0s, sz0s, sx1s, sz1s, lam, a.xrp[1],
+    shift_penta_line<KETA>(D, col, ix, iz, g, fcr, f0r, der, a.q[1], a.xrs[1], PX, PZ, sx0s, sz0s, sx1s, sz1s, lam, a.xrp[1],
                            a.tab[1], a.dst[1]);
 }
 
@@ -1079,7 +1111,7 @@ int launch_shift_penta(const Dims &D, const double2 *q2, const double2 *q1, doub
     a.xrp[k] = xrp[k];
     a.tab[k] = (const double4 *)tabs[k];
   }
-  dim3 grid((D.ncol + 255) / 256, 2);
+  dim3 grid((pair_lanes(D) + 255) / 256, 2);
   hipLaunchKernelGGL(k_shift_penta, grid, dim3(256), 0, s, D, a, fcr, f0r, der, PX, PZ, sx0, sz0, sx1, sz1, lambda);
   return hipGetLastError();
 }
