@@ -938,10 +938,13 @@ int launch_exact_shift(const Dims &D, int nq, void *const *q, const int *f32, do
 // elimination takes them.  Rows 0, 1 of x come from the reversed border
 // system, rows ny-2, ny-1 first from the record (for the ghost rows below row
 // 0), then from the back substitution.
-// three wavefronts per SIMD (at most 168 VGPRs) and loads two rows ahead in
-// each sweep: measured fastest (four wavefronts spill; deeper prefetch costs
-// the registers that the three sweeps' states need)
-constexpr int SP_WAVES = 3, SP_PF = 2;
+// Loads two rows ahead in each sweep.  The launch bound asks for two
+// wavefronts per SIMD; the kernel still fits in 155 VGPRs (three per SIMD),
+// and the compiler's schedule under this bound measured 4 % faster than under
+// a bound of three at nx = nz = 256, ny = 513.  Deeper prefetch ran slower.
+// One system alone (half the launch) takes 0.48 of the time of both: the
+// kernel is bound by HBM bandwidth, not by the second, partly filled round.
+constexpr int SP_WAVES = 2, SP_PF = 2;
 template <int K>
 __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, int ix, int iz, int g,
                                                  const D0Const &fcr,
