@@ -95,6 +95,10 @@ struct State {
   int64_t bytes_held = 0;
   bool prof = false;
   bool serial = false;  // the oldrhs shift on the main stream (HST_SERIAL=1)
+  // a time step leaves eta in V1 and dv/dy in V3 (every row) instead of u
+  // and w: the next step's transforms and right-hand sides read that form,
+  // and every other call restores u, w first (restore_uw)
+  bool compact = false;
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, ProfEntry> table;
@@ -284,8 +288,11 @@ void set_stress_hook(StressHook f, void *ctx) {
   g_hook_ctx = ctx;
 }
 
+static int restore_uw();
+
 bool state_view(StateView &v) {
   if (!S.live) return false;
+  if (restore_uw()) return false;
   v.mixed = S.mixed;
   v.D = S.D;
   v.deck = S.deck;
@@ -311,7 +318,7 @@ static int nonlinear_pass(int mode, bool want_cfl, double *cfl) {
   for (int row0 = 0; row0 < S.D.NYg; row0 += S.chunk) {
     const int nrows = std::min(S.chunk, S.D.NYg - row0);
     int rc = run("zift", bytes_zift(S.D, S.mixed, nrows),
-                 [&] { return launch_zift(S.D, S.mixed, V, S.Hv, row0, nrows, S.tables, S.stream); });
+                 [&] { return launch_zift(S.D, S.mixed, V, S.Hv, row0, nrows, S.tables, S.stream, S.compact); });
     if (rc) return rc;
     int npart = 0;
     double *part = S.d_part + npart_total;
@@ -370,6 +377,7 @@ static int buildrhs(const double ode[3], double deltat, double time, int mode, b
   double bytes = pts * (5.0 * 16.0 + 4.0 * ps);
   if (mode == 3) bytes += 6.0 * pts * ps;
   else if (mode == 4) bytes += 5.0 * pts * ps;
+  else if (mode == 5) bytes += 5.0 * pts * ps - pts * 16.0;  // no w
   else if (mode != 0) bytes = pts * (4.0 * 16.0 + 4.0 * ps + 3.0 * ps);
   return run(mode >= 3 ? "buildrhs" : (mode == 0 ? "buildrhs_prepare" : "buildrhs_group"), bytes, [&] {
     return launch_buildrhs(S.D, S.mixed, a, S.F[1], S.F[2], S.F[3], P, S.F[HST_RHS1], S.F[HST_RHS2], S.F[HST_OLD1],
@@ -489,7 +497,36 @@ static const void *const *penta_tables(double lambda) {
   return S.ptab[k];
 }
 
-static int linsolve(double lambda, double time) {
+// after the solves: dv/dy, and u, w from it and eta (compact: dv/dy into V3,
+// eta left in V1), then the ghost rows of both at the shifts (sx, sz)
+static int recover(bool compact, double sx, double sz) {
+  int rc;
+  if (compact) {
+    rc = run("dvdy", 4.0 * bytes_field(), [&] {
+      return launch_d0solve(S.D, 1, 0, S.F[HST_V2], S.F[HST_V3], nullptr, S.XR[0], S.d_f0, S.fc, S.d_der, S.d_dyl,
+                            sx, sz, nullptr, nullptr, S.stream);
+    });
+  } else {
+    rc = run("dy_recover", 6.0 * bytes_field(), [&] {
+      return launch_d0solve(S.D, 1, 1, S.F[HST_V2], S.F[HST_V1], S.F[HST_V3], S.XR[0], S.d_f0, S.fc, S.d_der,
+                            S.d_dyl, sx, sz, nullptr, nullptr, S.stream);
+    });
+  }
+  if (rc) return rc;
+  double2 *g13[2] = {S.F[HST_V1], S.F[HST_V3]};
+  if ((rc = ghosts(2, g13, sx, sz))) return rc;
+  S.compact = compact;
+  return 0;
+}
+
+static int restore_uw() {
+  if (!S.compact) return 0;
+  S.compact = false;
+  return run("uw_from_eta", 4.0 * S.D.NYg * S.D.ncol * 16.0,
+             [&] { return launch_uw_from_eta(S.D, S.F[HST_V1], S.F[HST_V3], S.stream); });
+}
+
+static int linsolve(double lambda, double time, bool compact = false) {
   double sx, sz;
   shear_shifts(time, sx, sz);
   const int kinds[2] = {HST_KIND_D2V, HST_KIND_ETA};
@@ -503,19 +540,13 @@ static int linsolve(double lambda, double time) {
   if (rc) return rc;
   double2 *g2[1] = {S.F[HST_V2]};
   if ((rc = ghosts(1, g2, sx, sz))) return rc;
-  rc = run("dy_recover", 6.0 * bytes_field(), [&] {
-    return launch_d0solve(S.D, 1, 1, S.F[HST_V2], S.F[HST_V1], S.F[HST_V3], S.XR[0], S.d_f0, S.fc, S.d_der, S.d_dyl,
-                          sx, sz, nullptr, nullptr, S.stream);
-  });
-  if (rc) return rc;
-  double2 *g13[2] = {S.F[HST_V1], S.F[HST_V3]};
-  return ghosts(2, g13, sx, sz);
+  return recover(compact, sx, sz);
 }
 
 // The exact shift of rhs1, rhs2 from t0 to t1 and linsolve at t1 with the
 // factor tables tabs: one fused pass for both (the shifted right-hand sides
 // stay in registers), then the ghosts of v, dv/dy with u and w, their ghosts.
-static int shift_and_linsolve(double lambda, double t0, double t1, const void *const *tabs) {
+static int shift_and_linsolve(double lambda, double t0, double t1, const void *const *tabs, bool compact) {
   double sx0, sz0, sx1, sz1;
   shear_shifts(t0, sx0, sz0);
   shear_shifts(t1, sx1, sz1);
@@ -529,13 +560,7 @@ static int shift_and_linsolve(double lambda, double t0, double t1, const void *c
   if (rc) return rc;
   double2 *g2[1] = {S.F[HST_V2]};
   if ((rc = ghosts(1, g2, sx1, sz1))) return rc;
-  rc = run("dy_recover", 6.0 * bytes_field(), [&] {
-    return launch_d0solve(S.D, 1, 1, S.F[HST_V2], S.F[HST_V1], S.F[HST_V3], S.XR[0], S.d_f0, S.fc, S.d_der, S.d_dyl,
-                          sx1, sz1, nullptr, nullptr, S.stream);
-  });
-  if (rc) return rc;
-  double2 *g13[2] = {S.F[HST_V1], S.F[HST_V3]};
-  return ghosts(2, g13, sx1, sz1);
+  return recover(compact, sx1, sz1);
 }
 
 // the main stream waits for the oldrhs shift of the side stream
@@ -551,7 +576,15 @@ static int check_live_nojoin() {
   return 0;
 }
 
+// the state as the reference holds it: u, v, w in V1..V3
 static int check_live() {
+  if (int rc = check_live_nojoin()) return rc;
+  if (int rc = join_side()) return rc;
+  return restore_uw();
+}
+
+// for calls that do not touch the velocity: the time step's form is kept
+static int check_live_keep() {
   if (int rc = check_live_nojoin()) return rc;
   return join_side();
 }
@@ -744,13 +777,13 @@ int hst_gpu_free(void) {
 }
 
 int hst_gpu_sync(void) {
-  if (int rc = check_live()) return rc;
+  if (int rc = check_live_keep()) return rc;
   HIPCHK(hipStreamSynchronize(S.stream));
   return 0;
 }
 
 int hst_gpu_grid(hst_grid_info *info, double *y, double *dyl, double *der) {
-  if (int rc = check_live()) return rc;
+  if (int rc = check_live_keep()) return rc;
   if (info) *info = S.g;
   if (y) std::memcpy(y, S.y.data(), S.y.size() * sizeof(double));
   if (dyl) std::memcpy(dyl, S.dyl.data(), S.dyl.size() * sizeof(double));
@@ -903,6 +936,11 @@ int hst_gpu_timestep(double deltat, double *time, double *cfl) {
   if (int rc = check_live_nojoin()) return rc;
   if (!time) return fail("time is NULL");
   double t = *time, c = 0.0;
+  // between substeps (and calls) eta and dv/dy stand for u and w unless a
+  // hook reads the velocity (closure, Stokes layer, damping) or the run is linear
+  const bool compact = !S.deck.linear && g_hook == nullptr && !stokes_active() && !damping_active();
+  if (!compact)
+    if (int rc = restore_uw()) return rc;
   for (int i = 0; i < 3; i++) {
     const double *ode = RK[i];
     int rc = 0;
@@ -920,7 +958,7 @@ int hst_gpu_timestep(double deltat, double *time, double *cfl) {
     const double dgx = S.deck.S * dt_sub, dgz = s2_integral(t, t + dt_sub);
     if ((rc = join_side())) return rc;  // the previous substep's oldrhs shift
     if ((rc = damping_update(t))) return rc;
-    rc = buildrhs(ode, deltat, t, S.deck.linear ? 0 : (six ? 3 : 4), fuse_shift, dgx, dgz, true);
+    rc = buildrhs(ode, deltat, t, S.deck.linear ? 0 : (six ? 3 : (S.compact ? 5 : 4)), fuse_shift, dgx, dgz, true);
     if (rc) return rc;
     // with the factor tables the rhs half of the exact shift is fused with the solves
     const double lam = ode[0] / deltat;
@@ -929,7 +967,7 @@ int hst_gpu_timestep(double deltat, double *time, double *cfl) {
     if (S.deck.exact_shift && (rc = exact_shift_split(dt_sub, t, fused))) return rc;
     const double t0 = t;
     t = t + dt_sub;
-    rc = fused ? shift_and_linsolve(lam, t0, t, tabs) : linsolve(lam, t);
+    rc = fused ? shift_and_linsolve(lam, t0, t, tabs, compact) : linsolve(lam, t, compact);
     if (rc) return rc;
     if ((rc = stokes_apply(t))) return rc;
   }
@@ -1085,7 +1123,7 @@ int hst_gpu_pressure(double time, int dst) {
 }
 
 int hst_gpu_profile(int enable) {
-  if (int rc = check_live()) return rc;
+  if (int rc = check_live_keep()) return rc;
   if (enable) {
     flush_profile();
     S.table.clear();

@@ -43,6 +43,23 @@ HSTD Cx<double> ld(const float2 *p) { float2 v = *p; return Cx<double>{(double)v
 HSTD void st(float2 *p, Cx<double> v) { *p = make_float2((float)v.x, (float)v.y); }
 HSTD Cx<float> ldf(const float2 *p) { float2 v = *p; return Cx<float>{v.x, v.y}; }
 
+// u and w of a Fourier column from x = dv/dy and eta (linsolve,
+// src/hst_equations.f90:303-324): u = (i alfa x - i beta eta) / k^2,
+// w = (i beta x + i alfa eta) / k^2; the (0,0) column: u = Re eta, w = Im eta.
+HSTD void recover_uw(double al, double be, double kk, bool m00, Cx<double> x, Cx<double> eta, Cx<double> &u,
+                     Cx<double> &w) {
+  if (m00) {
+    u = Cx<double>{eta.x, 0.0};
+    w = Cx<double>{eta.y, 0.0};
+  } else {
+    const double ik = 1.0 / kk;
+    const Cx<double> t = imul(al, x) - imul(be, eta);
+    const Cx<double> s = imul(be, x) + imul(al, eta);
+    u = Cx<double>{t.x * ik, t.y * ik};
+    w = Cx<double>{s.x * ik, s.y * ik};
+  }
+}
+
 #undef HSTD
 
 // Workgroups are dealt to the 8 XCDs round-robin by their linear id and every

@@ -157,39 +157,99 @@ __device__ __forceinline__ void xcd_tile(int &tile, int &row, int &comp) {
 
 // z inverse transform of the three velocity components for the rows of a
 // chunk.  grid (x tiles of G modes, rows, component); output Hv[c][row][zp][ix].
-template <typename R, int NZD>
+// CMP: V1 holds eta and V3 dv/dy (the time step's form between its
+// substeps); grid (npt + nvt, rows): the first npt = ceil(NXm / (G/2)) blocks
+// of a row recover u and w of G/2 modes as they load them (recover_uw) and
+// transform the G lines, the other nvt = ceil(NXm / G) transform v of G modes.
+template <typename R, int NZD, bool CMP>
 __global__ void __launch_bounds__((ZCfg<R, NZD, false>::NT))
     k_zift(const double2 *__restrict__ V1, const double2 *__restrict__ V2, const double2 *__restrict__ V3,
            Cx<R> *__restrict__ Hv, Dims D, int row0, int nrows, const Cx<R> *__restrict__ tw) {
   constexpr int G = ZCfg<R, NZD, false>::G, T = ZCfg<R, NZD, false>::T, LS = ZCfg<R, NZD, false>::LS;
+  static_assert(!CMP || G % 2 == 0, "the u, w blocks split a tile in two");
+  constexpr int HG = G / 2, NT = ZCfg<R, NZD, false>::NT, PER = (NZD + NT - 1) / NT;
+  int tile, rowc, c;
+  xcd_tile(tile, rowc, c);
+  const int npt = (D.NXm + HG - 1) / HG;
+  const bool pair = CMP && tile < npt;
+  if (CMP && !pair) {
+    tile -= npt;
+    c = 1;
+  }
   __shared__ Cx<R> lds[G * LS];
   __shared__ Cx<R> twl[ztw_lds<R>() ? NZD : 1];  // the twiddles: lanes of a stage read scattered entries
   if constexpr (ztw_lds<R>())
     for (int k = threadIdx.x; k < NZD; k += blockDim.x) twl[k] = tw[k];
   const Cx<R> *twp = ztw_lds<R>() ? twl : tw;
-  int tile, rowc, c;
-  xcd_tile(tile, rowc, c);
-  const int row = row0 + rowc, ix0 = tile * G;
-  const double2 *src = (c == 0 ? V1 : (c == 1 ? V2 : V3)) + (size_t)row * D.ncol + (size_t)ix0 * D.NZm;
-  const int nz = D.nz;
-  for (int j = 0; j < G; j++) {
-    const bool ok = ix0 + j < D.NXm;
-    for (int k = threadIdx.x; k < NZD; k += blockDim.x) {
-      Cx<R> v{(R)0, (R)0};
-      if (ok) {
-        if (k <= nz) v = cx_from<R>(src[(size_t)j * D.NZm + k + nz]);
-        else if (k >= NZD - nz) v = cx_from<R>(src[(size_t)j * D.NZm + (k - NZD) + nz]);
+  const int row = row0 + rowc, nz = D.nz;
+  const size_t rb = (size_t)row * D.ncol;
+  if (!pair) {
+    const int cc = CMP ? 1 : c, ix0 = tile * G;
+    const double2 *src = (cc == 0 ? V1 : (cc == 1 ? V2 : V3)) + rb + (size_t)ix0 * D.NZm;
+    for (int j = 0; j < G; j++) {
+      const bool ok = ix0 + j < D.NXm;
+      for (int k = threadIdx.x; k < NZD; k += blockDim.x) {
+        Cx<R> v{(R)0, (R)0};
+        if (ok) {
+          if (k <= nz) v = cx_from<R>(src[(size_t)j * D.NZm + k + nz]);
+          else if (k >= NZD - nz) v = cx_from<R>(src[(size_t)j * D.NZm + (k - NZD) + nz]);
+        }
+        lds[j * LS + k] = v;
       }
-      lds[j * LS + k] = v;
     }
-  }
-  __syncthreads();
-  const int g = threadIdx.x / T, t = threadIdx.x % T;
-  fft_line<R, NZD, T, +1>(lds + g * LS, t, twp);
-  Cx<R> *dst = Hv + ((size_t)c * nrows + rowc) * NZD * D.NXp;
-  for (int e = threadIdx.x; e < G * NZD; e += blockDim.x) {
-    const int j = e % G, zp = e / G;
-    if (ix0 + j < D.NXm) dst[hoff<R, 0>(NZD, D.NXp, zp, ix0 + j)] = lds[j * LS + zp];
+    __syncthreads();
+    const int g = threadIdx.x / T, t = threadIdx.x % T;
+    fft_line<R, NZD, T, +1>(lds + g * LS, t, twp);
+    Cx<R> *dst = Hv + ((size_t)cc * nrows + rowc) * NZD * D.NXp;
+    for (int e = threadIdx.x; e < G * NZD; e += blockDim.x) {
+      const int j = e % G, zp = e / G;
+      if (ix0 + j < D.NXm) dst[hoff<R, 0>(NZD, D.NXp, zp, ix0 + j)] = lds[j * LS + zp];
+    }
+  } else {
+    const int ix0 = tile * HG;
+    // every load of the block's lines first, then the recovery
+    Cx<double> xv[HG][PER], ev[HG][PER];
+#pragma unroll
+    for (int j = 0; j < HG; j++) {
+      const size_t c0 = rb + (size_t)(ix0 + j) * D.NZm + nz;  // the column (ix, 0)
+#pragma unroll
+      for (int q = 0; q < PER; q++) {
+        const int k = threadIdx.x + q * NT;
+        xv[j][q] = ev[j][q] = Cx<double>{0.0, 0.0};
+        if (ix0 + j < D.NXm && k < NZD && (k <= nz || k >= NZD - nz)) {
+          const int iz = k <= nz ? k : k - NZD;
+          xv[j][q] = ld(V3 + c0 + iz);
+          ev[j][q] = ld(V1 + c0 + iz);
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < HG; j++) {
+      const int ix = ix0 + j;
+      const double al = D.alfa0 * ix;
+#pragma unroll
+      for (int q = 0; q < PER; q++) {
+        const int k = threadIdx.x + q * NT;
+        if (k < NZD) {
+          const int iz = k <= nz ? k : k - NZD;
+          const double be = D.beta0 * iz;
+          Cx<double> ud, wd;
+          recover_uw(al, be, al * al + be * be, ix == 0 && iz == 0, xv[j][q], ev[j][q], ud, wd);
+          lds[j * LS + k] = Cx<R>{(R)ud.x, (R)ud.y};
+          lds[(HG + j) * LS + k] = Cx<R>{(R)wd.x, (R)wd.y};
+        }
+      }
+    }
+    __syncthreads();
+    const int g = threadIdx.x / T, t = threadIdx.x % T;
+    fft_line<R, NZD, T, +1>(lds + g * LS, t, twp);
+    for (int h = 0; h < 2; h++) {
+      Cx<R> *dst = Hv + ((size_t)(2 * h) * nrows + rowc) * NZD * D.NXp;
+      for (int e = threadIdx.x; e < HG * NZD; e += blockDim.x) {
+        const int j = e % HG, zp = e / HG;
+        if (ix0 + j < D.NXm) dst[hoff<R, 0>(NZD, D.NXp, zp, ix0 + j)] = lds[(h * HG + j) * LS + zp];
+      }
+    }
   }
 }
 
@@ -522,11 +582,17 @@ __global__ void __launch_bounds__(64, (xline_min_waves<R, M>()))
 
 template <typename R, int NZD>
 static int run_zift(const Dims &D, const double2 *const V[3], void *Hv, int row0, int nrows, const Tables<R> &tb,
-                    hipStream_t s) {
+                    int compact, hipStream_t s) {
   typedef ZCfg<R, NZD, false> C;
-  dim3 grid((D.NXm + C::G - 1) / C::G, nrows, 3);
-  hipLaunchKernelGGL((k_zift<R, NZD>), grid, dim3(C::NT), 0, s, V[0], V[1], V[2], (Cx<R> *)Hv, D, row0, nrows,
-                     tb.twz);
+  if (compact) {
+    dim3 grid((D.NXm + C::G / 2 - 1) / (C::G / 2) + (D.NXm + C::G - 1) / C::G, nrows, 1);
+    hipLaunchKernelGGL((k_zift<R, NZD, true>), grid, dim3(C::NT), 0, s, V[0], V[1], V[2], (Cx<R> *)Hv, D, row0,
+                       nrows, tb.twz);
+  } else {
+    dim3 grid((D.NXm + C::G - 1) / C::G, nrows, 3);
+    hipLaunchKernelGGL((k_zift<R, NZD, false>), grid, dim3(C::NT), 0, s, V[0], V[1], V[2], (Cx<R> *)Hv, D, row0,
+                       nrows, tb.twz);
+  }
   return hipGetLastError();
 }
 
@@ -582,9 +648,9 @@ static int run_xpass(const Dims &D, int mode, const void *Hv, void *Hp, int row0
 
 template <typename R>
 static int zift_dispatch(const Dims &D, const double2 *const V[3], void *Hv, int row0, int nrows, const Tables<R> &tb,
-                         hipStream_t s) {
+                         int compact, hipStream_t s) {
   switch (D.nzd) {
-#define CASE(n) case n: return run_zift<R, n>(D, V, Hv, row0, nrows, tb, s);
+#define CASE(n) case n: return run_zift<R, n>(D, V, Hv, row0, nrows, tb, compact, s);
     HST_FOR_SIZES(CASE)
 #undef CASE
   }
@@ -614,10 +680,10 @@ static int xpass_dispatch(const Dims &D, int mode, const void *Hv, void *Hp, int
 }
 
 int launch_zift(const Dims &D, int mixed, const double2 *const V[3], void *Hv, int row0, int nrows,
-                const void *tables, hipStream_t s) {
+                const void *tables, hipStream_t s, int compact) {
   const TableSet *t = (const TableSet *)tables;
-  return mixed ? zift_dispatch<float>(D, V, Hv, row0, nrows, t->f, s)
-               : zift_dispatch<double>(D, V, Hv, row0, nrows, t->d, s);
+  return mixed ? zift_dispatch<float>(D, V, Hv, row0, nrows, t->f, compact, s)
+               : zift_dispatch<double>(D, V, Hv, row0, nrows, t->d, compact, s);
 }
 
 int launch_xpass(const Dims &D, int mixed, int mode, const void *Hv, void *Hp, int row0, int nrows,

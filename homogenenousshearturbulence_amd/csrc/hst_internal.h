@@ -64,8 +64,9 @@ void free_fft_tables(void *tables);
 // or, mode 1, the velocities times factor, or, mode 2, nothing but the CFL
 // partial maxima; x forward into Hp), k_zfft (z forward of np spectra into
 // P0 + p*pstride, the spectral layout).
+// compact: V[0] holds eta and V[2] dv/dy; u and w are recovered on the way in
 int launch_zift(const Dims &D, int mixed, const double2 *const V[3], void *Hv, int row0, int nrows,
-                const void *tables, hipStream_t s);
+                const void *tables, hipStream_t s, int compact = 0);
 int launch_xpass(const Dims &D, int mixed, int mode, const void *Hv, void *Hp, int row0, int nrows,
                  const void *tables, const double *dyl, double *cfl_part, int want_cfl, int *n_part, hipStream_t s);
 int launch_zfft(const Dims &D, int mixed, int np, const void *Hp, void *P0, size_t pstride, int row0, int nrows,
@@ -89,6 +90,8 @@ struct RhsArgs {
 // oldrhs (O1, O2) is FP32 (float2) in the mixed form, FP64 otherwise.
 int launch_phase4(const Dims &D, int mixed, double2 *R1, double2 *R2, void *O1, void *O2, const double *y,
                   double dgx, double dgz, hipStream_t s);
+// u, w from eta (V1) and dv/dy (V3) in place, every row
+int launch_uw_from_eta(const Dims &D, double2 *V1, double2 *V3, hipStream_t s);
 int launch_buildrhs(const Dims &D, int mixed, const RhsArgs &a, const double2 *V1, const double2 *V2,
                     const double2 *V3, const void *const P[6], double2 *R1, double2 *R2, void *O1, void *O2,
                     const double *der, const double *y, hipStream_t s);

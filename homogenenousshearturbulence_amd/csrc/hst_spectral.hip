@@ -66,8 +66,11 @@ __global__ void __launch_bounds__(256)
                double2 *__restrict__ R1, double2 *__restrict__ R2, typename PTraits<PR>::T2 *__restrict__ O1,
                typename PTraits<PR>::T2 *__restrict__ O2, const double *__restrict__ der,
                const double *__restrict__ y, int rch) {
-  constexpr bool NONLIN = MODE != 0, PREP = (MODE == 0 || MODE == 3 || MODE == 4);
-  constexpr int Q0 = (MODE == 2 ? 3 : 0), Q1 = (MODE == 1 ? 3 : (MODE == 4 ? 5 : 6));
+  // FIVE: the five combinations; CMP (MODE 5): V1 holds eta and V3 dv/dy (the
+  // time step's own form between substeps), so the eta equation's f is V1
+  constexpr bool FIVE = (MODE == 4 || MODE == 5), CMP = (MODE == 5);
+  constexpr bool NONLIN = MODE != 0, PREP = (MODE == 0 || MODE == 3 || FIVE);
+  constexpr int Q0 = (MODE == 2 ? 3 : 0), Q1 = (MODE == 1 ? 3 : (FIVE ? 5 : 6));
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= D.ncol) return;
   int ix, iz;
@@ -86,7 +89,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int k = 1; k < 5; k++) {
     const size_t r = (size_t)(iy0 - 2 + k - 1 + 2) * nc + col;
-    if constexpr (PREP) { u[k] = ld(V1 + r); v[k] = ld(V2 + r); w[k] = ld(V3 + r); }
+    if constexpr (PREP) { u[k] = ld(V1 + r); v[k] = ld(V2 + r); if constexpr (!CMP) w[k] = ld(V3 + r); }
     if constexpr (NONLIN) {
 #pragma unroll
       for (int q = Q0; q < Q1; q++) p[q][k] = ldp<PR>(P0 + q * pstride + r);
@@ -96,7 +99,7 @@ __global__ void __launch_bounds__(256)
   for (int iy = iy0; iy < iy1; iy++) {
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      if constexpr (PREP) { u[k] = u[k + 1]; v[k] = v[k + 1]; w[k] = w[k + 1]; }
+      if constexpr (PREP) { u[k] = u[k + 1]; v[k] = v[k + 1]; if constexpr (!CMP) w[k] = w[k + 1]; }
       if constexpr (NONLIN) {
 #pragma unroll
         for (int q = Q0; q < Q1; q++) p[q][k] = p[q][k + 1];
@@ -104,7 +107,7 @@ __global__ void __launch_bounds__(256)
     }
     {
       const size_t r = (size_t)(iy + 2 + 2) * nc + col;
-      if constexpr (PREP) { u[4] = ld(V1 + r); v[4] = ld(V2 + r); w[4] = ld(V3 + r); }
+      if constexpr (PREP) { u[4] = ld(V1 + r); v[4] = ld(V2 + r); if constexpr (!CMP) w[4] = ld(V3 + r); }
       if constexpr (NONLIN) {
 #pragma unroll
         for (int q = Q0; q < Q1; q++) p[q][4] = ldp<PR>(P0 + q * pstride + r);
@@ -143,12 +146,12 @@ __global__ void __launch_bounds__(256)
       const double d0 = DER(der, iy, 0, j - 2), d2 = DER(der, iy, 2, j - 2);
       const double helm = d2 - kk * d0;
       if (m00) {
-        const C f{u[j].x, w[j].x};
+        const C f = CMP ? u[j] : C{u[j].x, w[j].x};
         unk1 = unk1 + d0 * f;
         imp1 = imp1 + (D.ni * d2) * f;
         if (a.fy) force = force + d0 * a.fy[iy + j];  // the layer's body force on the mean w, D0-weighted
       } else {
-        const C f = imul(be, u[j]) - imul(al, w[j]);
+        const C f = CMP ? u[j] : imul(be, u[j]) - imul(al, w[j]);
         unk1 = unk1 + d0 * f;
         imp1 = imp1 + (D.ni * helm) * f;
       }
@@ -160,7 +163,7 @@ __global__ void __launch_bounds__(256)
     r1 = C{r1.x / a.deltat, r1.y / a.deltat} + imp1 - a.ode2 * o1old + a.ode1 * expl;
     o1 = expl;
     }
-    if constexpr (MODE == 4) {
+    if constexpr (FIVE) {
       // buildrhs (:169-207) summed over the six products and regrouped by
       // the combinations P0 = vv - uu, P1 = vv - ww, P2 = uv, P3 = vw, P4 = uw:
       //   eta: alfa beta D0 (P1 - P0) + (beta^2 - alfa^2) D0 P4 - i beta D1 P2 + i alfa D1 P3
@@ -253,7 +256,9 @@ int launch_buildrhs(const Dims &D, int mixed, const RhsArgs &a, const double2 *V
   const int rch = 64;  // rows per lane: the 5-row windows re-read 4 rows per chunk
   dim3 grid((D.ncol + 255) / 256, (D.ny + rch - 1) / rch);
   const size_t pstride = (size_t)D.NYg * D.ncol;
-  const int mode = a.nonlinear;  // 0 prepare, 1/2 one group, 3 all, 4 all from the five combinations
+  // 0 prepare, 1/2 one group, 3 all, 4 all from the five combinations, 5 as 4
+  // with eta and dv/dy in V1 and V3
+  const int mode = a.nonlinear;
   const bool sh = a.shift != 0;
 #define L(PR, MD, SH)                                                                                            \
   hipLaunchKernelGGL((k_buildrhs<PR, MD, SH>), grid, dim3(256), 0, s, D, a, V1, V2, V3,                          \
@@ -265,11 +270,34 @@ int launch_buildrhs(const Dims &D, int mixed, const RhsArgs &a, const double2 *V
     case 1: L(PR, 1, false); break;                                \
     case 2: L(PR, 2, false); break;                                \
     case 4: if (sh) L(PR, 4, true); else L(PR, 4, false); break;   \
+    case 5: if (sh) L(PR, 5, true); else L(PR, 5, false); break;   \
     default: if (sh) L(PR, 3, true); else L(PR, 3, false); break;  \
   }
   if (mixed) { M(float) } else { M(double) }
 #undef M
 #undef L
+  return hipGetLastError();
+}
+
+// u, w from eta (in V1) and dv/dy (in V3) in place, on every row (ghost rows
+// included): the time step leaves the velocity in that form between its
+// calls, and this restores u, v, w.
+__global__ void __launch_bounds__(256) k_uw_from_eta(Dims D, double2 *__restrict__ V1, double2 *__restrict__ V3) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= D.ncol) return;
+  int ix, iz;
+  col_ix_iz(D, col, ix, iz);
+  const double al = D.alfa0 * ix, be = D.beta0 * iz, kk = al * al + be * be;
+  const size_t o = (size_t)blockIdx.y * D.ncol + col;
+  C u, w;
+  recover_uw(al, be, kk, ix == 0 && iz == 0, ld(V3 + o), ld(V1 + o), u, w);
+  st(V1 + o, u);
+  st(V3 + o, w);
+}
+
+int launch_uw_from_eta(const Dims &D, double2 *V1, double2 *V3, hipStream_t s) {
+  dim3 grid((D.ncol + 255) / 256, D.NYg);
+  hipLaunchKernelGGL(k_uw_from_eta, grid, dim3(256), 0, s, D, V1, V3);
   return hipGetLastError();
 }
 
@@ -734,15 +762,10 @@ __global__ void __launch_bounds__(256)
       if constexpr (OUT == 0) {
         st(dst + o, x);
       } else if constexpr (OUT == 1) {
-        if (m00) {
-          st(dst + o, C{eta.x, 0.0});
-          st(dst3 + o, C{eta.y, 0.0});
-        } else {
-          const C t = imul(al, x) - imul(be, eta);
-          const C ww = imul(be, x) + imul(al, eta);
-          st(dst + o, C{t.x / kk, t.y / kk});
-          st(dst3 + o, C{ww.x / kk, ww.y / kk});
-        }
+        C uu, ww;
+        recover_uw(al, be, kk, m00, x, eta, uu, ww);
+        st(dst + o, uu);
+        st(dst3 + o, ww);
       } else if (!skip) {
         const double wgt = (ix == 0 ? dyl[i] : 2.0 * dyl[i]);
         const double q = wgt * (x.x * x.x + x.y * x.y);

@@ -148,7 +148,13 @@ int hst_gpu_linsolve(double lambda, double time);
 
 /* timestep() (src/hst_equations.f90:56-87): one RK3 step of length deltat.
  * *time is advanced in FP64 by 2/RK_rai(1,i)*deltat per substep, as the
- * reference does; *cfl receives the substep-3 CFL number (may be NULL). */
+ * reference does; *cfl receives the substep-3 CFL number (may be NULL).
+ * Without a closure, Stokes layer or damping (and not linear) the step keeps
+ * eta and dv/dy in the device arrays of u and w between substeps and between
+ * consecutive calls; every other entry point that touches the velocity
+ * (download, outstats, the solves, ...) restores u, w first, so the state seen
+ * through this interface is the reference's.  hst_gpu_sync, hst_gpu_grid and
+ * hst_gpu_profile leave it as it is. */
 int hst_gpu_timestep(double deltat, double *time, double *cfl);
 
 /* outstats (src/hst_stats.f90:136-270) at `time` (the wrap phase of its dv/dy

@@ -246,6 +246,39 @@ def test_one_step(O, exact, precision, tol):
         g.close()
 
 
+@pytest.mark.parametrize("exact", [False, True])
+@pytest.mark.parametrize("precision,tol", [(hst.FP64, 1e-12), (hst.MIXED, 1e-5)])
+def test_steps_with_reads_between(O, exact, precision, tol):
+    """Between its calls the time step keeps eta and dv/dy in place of u and w;
+    every other call that touches the velocity restores u, v, w first.  Four
+    steps with the velocity read after each (a restore every step) and four
+    with only sync between them (the form kept across calls), both against
+    the oracle and against each other."""
+    d = dataclasses.replace(read_input(deck_path("small")), exact_shift=exact)
+    o = O.run_deck(d, nsteps=4)
+    runs = []
+    for read_between in (True, False):
+        g = hst.Solver(d, precision)
+        try:
+            g.time = d.time
+            g.initial_field()
+            g.fill_all_ghosts()
+            for _ in range(4):
+                g.timestep(d.deltat)
+                if read_between:
+                    assert np.all(np.isfinite(g.V))
+                else:
+                    g.sync()
+            assert g.time == o.time
+            runs.append(g.V)
+            assert np.array_equal(g.V, runs[-1])  # a second read: nothing left to restore
+        finally:
+            g.close()
+    for V in runs:
+        assert rel(V, o.V) <= tol
+    assert rel(runs[0], runs[1]) <= tol
+
+
 @pytest.mark.parametrize("name", ["small", "small_s2x"])
 def test_golden_fields(name):
     """tests/regression.sh: 50 steps vs the reference's stored fields at 1e-10."""
