@@ -275,14 +275,26 @@ __global__ void __launch_bounds__((ZCfg<R, NZD, true>::NT))
   }
   __syncthreads();
   const int g = threadIdx.x / T, t = threadIdx.x % T;
-  fft_line<R, NZD, T, -1>(lds + g * LS, t, twp);
-  Cx<R> *dst = P0 + (size_t)p * pstride + (size_t)row * D.ncol + (size_t)ix0 * D.NZm;
   const int nz = D.nz;
-  for (int j = 0; j < G; j++) {
-    if (ix0 + j >= D.NXm) break;
-    for (int izm = threadIdx.x; izm < D.NZm; izm += blockDim.x) {
-      const int iz = izm - nz;
-      dst[(size_t)j * D.NZm + izm] = lds[j * LS + (iz >= 0 ? iz : NZD + iz)];
+  if constexpr (sizeof(R) == 4) {
+    // FP32: the last stage stores the kept modes of line g straight to P (the
+    // modes of one x mode are consecutive there), measured 4 % faster at
+    // nzd = 768; FP64 measured slower that way and goes through LDS
+    Cx<R> *dst = P0 + (size_t)p * pstride + (size_t)row * D.ncol + (size_t)(ix0 + g) * D.NZm + nz;
+    const bool okg = ix0 + g < D.NXm;
+    fft_line_emit<R, NZD, T, -1>(lds + g * LS, t, twp, [&](int k, const Cx<R> &val) {
+      if (okg && k <= nz) dst[k] = val;
+      else if (okg && k >= NZD - nz) dst[k - NZD] = val;
+    });
+  } else {
+    fft_line<R, NZD, T, -1>(lds + g * LS, t, twp);
+    Cx<R> *dst = P0 + (size_t)p * pstride + (size_t)row * D.ncol + (size_t)ix0 * D.NZm;
+    for (int j = 0; j < G; j++) {
+      if (ix0 + j >= D.NXm) break;
+      for (int izm = threadIdx.x; izm < D.NZm; izm += blockDim.x) {
+        const int iz = izm - nz;
+        dst[(size_t)j * D.NZm + izm] = lds[j * LS + (iz >= 0 ? iz : NZD + iz)];
+      }
     }
   }
 }

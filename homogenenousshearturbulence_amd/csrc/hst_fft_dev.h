@@ -99,9 +99,12 @@ __device__ __forceinline__ void dft(Cx<R> *v) {
 // x[q + s (n1 + L n2)], n2 < r, and writes its DFT times w_len^(n1 k2) to
 // y[q + s k2 + s r n1].  In place: every thread holds its butterflies in
 // registers across the barrier.
-template <typename R, int N, int T, int SIGN, int RAD, int S>
-__device__ __forceinline__ void fft_stage(Cx<R> *line, int t, const Cx<R> *__restrict__ tw) {
+// The last stage (S RAD = N, so n1 = 0) has no twiddles; with EMIT it hands
+// each result to emit(k, value) (k the output index) instead of storing it.
+template <typename R, int N, int T, int SIGN, int RAD, int S, bool EMIT = false, class F = int>
+__device__ __forceinline__ void fft_stage(Cx<R> *line, int t, const Cx<R> *__restrict__ tw, F emit = 0) {
   constexpr int len = N / S, L = len / RAD, B = N / RAD, K = (B + T - 1) / T;
+  constexpr bool LAST = (S * RAD == N);
   Cx<R> v[K][RAD];
 #pragma unroll
   for (int k = 0; k < K; k++) {
@@ -112,23 +115,32 @@ __device__ __forceinline__ void fft_stage(Cx<R> *line, int t, const Cx<R> *__res
       for (int n2 = 0; n2 < RAD; n2++) v[k][n2] = line[q + S * (n1 + L * n2)];
     }
   }
-  __syncthreads();
+  if constexpr (!EMIT) __syncthreads();
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const int b = t + T * k;
     if (B % T == 0 || b < B) {
       const int q = b % S, n1 = b / S;
       dft<R, SIGN, RAD>(v[k]);
-      line[q + S * RAD * n1] = v[k][0];
+      if constexpr (EMIT) {
 #pragma unroll
-      for (int k2 = 1; k2 < RAD; k2++) {
-        Cx<R> w = tw[n1 * k2 * S];
-        if (SIGN > 0) w.y = -w.y;
-        line[q + S * k2 + S * RAD * n1] = v[k][k2] * w;
+        for (int k2 = 0; k2 < RAD; k2++) emit(q + S * k2, v[k][k2]);
+      } else {
+        line[q + S * RAD * n1] = v[k][0];
+#pragma unroll
+        for (int k2 = 1; k2 < RAD; k2++) {
+          if constexpr (LAST) {
+            line[q + S * k2] = v[k][k2];
+          } else {
+            Cx<R> w = tw[n1 * k2 * S];
+            if (SIGN > 0) w.y = -w.y;
+            line[q + S * k2 + S * RAD * n1] = v[k][k2] * w;
+          }
+        }
       }
     }
   }
-  __syncthreads();
+  if constexpr (!EMIT) __syncthreads();
 }
 
 template <typename R, int N, int T, int SIGN, int ST = 0, int S = 1>
@@ -137,6 +149,19 @@ __device__ __forceinline__ void fft_line(Cx<R> *line, int t, const Cx<R> *__rest
     constexpr int RAD = radix_at(N, ST);
     fft_stage<R, N, T, SIGN, RAD, S>(line, t, tw);
     fft_line<R, N, T, SIGN, ST + 1, S * RAD>(line, t, tw);
+  }
+}
+
+// fft_line whose last stage hands its results to emit(k, value) instead of
+// writing them back to the line (which it leaves as the stage before left it)
+template <typename R, int N, int T, int SIGN, class F, int ST = 0, int S = 1>
+__device__ __forceinline__ void fft_line_emit(Cx<R> *line, int t, const Cx<R> *__restrict__ tw, F emit) {
+  constexpr int RAD = radix_at(N, ST);
+  if constexpr (ST + 1 < n_stages(N)) {
+    fft_stage<R, N, T, SIGN, RAD, S>(line, t, tw);
+    fft_line_emit<R, N, T, SIGN, F, ST + 1, S * RAD>(line, t, tw, emit);
+  } else {
+    fft_stage<R, N, T, SIGN, RAD, S, true>(line, t, tw, emit);
   }
 }
 
