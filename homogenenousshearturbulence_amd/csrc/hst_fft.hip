@@ -427,6 +427,34 @@ struct XL {
 
 __device__ __forceinline__ int padx(int p) { return p + (p >> 3); }
 
+// The x pass's twiddles in LDS, stage by stage: a stage of stride S and
+// radix RAD (not the last) multiplies output k2 of butterfly n1 by
+// w^(n1 k2 S), w = exp(-2 pi i / M); its table holds them [k2 - 1][n1], so
+// that the lanes of a wavefront, whose n1 are consecutive (or equal), read
+// neighbouring words (read as tw[n1 k2 S], a stage's lanes hit one bank).
+template <int M, int ST = 0, int S = 1>
+__host__ __device__ constexpr int xtw_off(int st) {
+  if constexpr (ST + 1 >= n_stages(M)) {
+    return 0;
+  } else {
+    constexpr int RAD = radix_at(M, ST), Ls = M / (S * RAD);
+    return st <= ST ? 0 : (RAD - 1) * Ls + xtw_off<M, ST + 1, S * RAD>(st);
+  }
+}
+template <int M>
+__host__ __device__ constexpr int xtw_size() { return xtw_off<M>(n_stages(M)); }
+// the exponent n1 k2 S of entry i of the staged table
+template <int M, int ST = 0, int S = 1>
+__device__ __forceinline__ int xtw_exponent(int i) {
+  if constexpr (ST + 1 >= n_stages(M)) {
+    return 0;
+  } else {
+    constexpr int RAD = radix_at(M, ST), Ls = M / (S * RAD), n = (RAD - 1) * Ls;
+    if (i < n) return (i % Ls) * (1 + i / Ls) * S;
+    return xtw_exponent<M, ST + 1, S * RAD>(i - n);
+  }
+}
+
 template <typename R, int M, int SIGN, int ST, int S>
 __device__ __forceinline__ void xl_fft(Cx<R> (&v)[XL<M>::E], Cx<R> *L, int t, const Cx<R> *__restrict__ tw) {
   constexpr int T = XL<M>::T, E = XL<M>::E;
@@ -453,9 +481,10 @@ __device__ __forceinline__ void xl_fft(Cx<R> (&v)[XL<M>::E], Cx<R> *L, int t, co
       dft<R, SIGN, RAD>(c[j]);
       if constexpr (!LAST) {
         const int b = t + T * j, n1 = b / S;
+        constexpr int OFF = xtw_off<M>(ST);
 #pragma unroll
         for (int k2 = 1; k2 < RAD; k2++) {
-          Cx<R> w = tw[n1 * k2 * S];
+          Cx<R> w = tw[OFF + (k2 - 1) * Ls + n1];
           if (SIGN > 0) w.y = -w.y;
           c[j][k2] = c[j][k2] * w;
         }
@@ -514,8 +543,8 @@ __global__ void __launch_bounds__(64, (xline_min_waves<R, M>()))
             int want_cfl, int row0) {
   constexpr int T = XL<M>::T, E = XL<M>::E, LPB = XL<M>::LPB, PL = XL<M>::PL;
   __shared__ Cx<R> lds[LPB * PL];
-  __shared__ Cx<R> twl[M];  // twiddles: the lanes of a stage read scattered entries
-  for (int k = threadIdx.x; k < M; k += 64) twl[k] = twx[k];
+  __shared__ Cx<R> twl[xtw_size<M>() > 0 ? xtw_size<M>() : 1];  // the staged twiddles
+  for (int k = threadIdx.x; k < xtw_size<M>(); k += 64) twl[k] = twx[xtw_exponent<M>(k)];
   __syncthreads();
   const int g = threadIdx.x / T, t = threadIdx.x % T;
   const int NZD = D.nzd, nx = D.nx;
