@@ -396,8 +396,8 @@ static int exact_shift(double dt_sub, double time) {
     return rc;
   void *q[4] = {S.F[HST_RHS1], S.F[HST_RHS2], S.F[HST_OLD1], S.F[HST_OLD2]};
   const int f32[4] = {0, 0, S.mixed, S.mixed};
-  // per array: q in, the solution x out and back in (FP64), q out
-  const double qb = 2.0 * 2.0 * 16.0 + 2.0 * 2.0 * (S.mixed ? 8.0 : 16.0) + 4.0 * 2.0 * 16.0;
+  // per array q in twice and out once (rhs FP64, oldrhs FP32 in the mixed form)
+  const double qb = 2.0 * 3.0 * 16.0 + 2.0 * 3.0 * (S.mixed ? 8.0 : 16.0);
   return run("exact_shift", qb / 16.0 * bytes_field(), [&] {
     return launch_exact_shift(S.D, 4, q, f32, S.XR, S.d_f0, S.fc, S.d_der, S.PX, S.PZ, sx0, sz0, sx1, sz1,
                               S.stream);
@@ -425,7 +425,8 @@ static int exact_shift_split(double dt_sub, double time, bool fused = false) {
   HIPCHK(hipEventRecord(S.ev_fork, S.stream));
   int rc = 0;
   if (!fused) {
-    rc = run("exact_shift", 2.0 * (2.0 * 16.0 + 2.0 * 16.0) / 16.0 * bytes_field(), [&] {
+    // per array q in twice (the forward sweep, then the recomputed chunks) and out once
+    rc = run("exact_shift", 2.0 * 3.0 * bytes_field(), [&] {
       return launch_exact_shift(S.D, 2, qr, fr, xr, S.d_f0, S.fc, S.d_der, S.PX, S.PZ, sx0, sz0, sx1, sz1,
                                 S.stream);
     });
@@ -437,8 +438,8 @@ static int exact_shift_split(double dt_sub, double time, bool fused = false) {
   const bool serial = S.prof || S.serial;
   hipStream_t st = serial ? S.stream : S.side;
   if (!serial) HIPCHK(hipStreamWaitEvent(S.side, S.ev_fork, 0));
-  // an FP32 array keeps its eliminated rows in FP32 too: q in, XR out and in, q out
-  rc = run_on(st, "exact_shift_old", 2.0 * (4.0 * os) / 16.0 * bytes_field(), [&] {
+  // per array q in twice and out once, FP32 in the mixed form
+  rc = run_on(st, "exact_shift_old", 2.0 * (3.0 * os) / 16.0 * bytes_field(), [&] {
     return launch_exact_shift(S.D, 2, qo, fo, S.XRo, S.d_f0, S.fc, S.d_der, S.PX, S.PZ, sx0, sz0, sx1, sz1, st);
   });
   if (rc) return rc;

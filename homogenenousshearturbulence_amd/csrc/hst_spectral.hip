@@ -835,7 +835,7 @@ int launch_d0solve(const Dims &D, int rhs_mode, int out_mode, const double2 *src
 // re-weighted rows are emitted in the same sweep.
 struct QList {
   void *q[4];      // double2, or float2 where f32 is set
-  double2 *xr[4];  // the D0 solution of each array, ny - 2 rows of ncol
+  double2 *xr[4];  // per array a workspace of ny - 2 rows of ncol (the checkpoints use 1/4 of it)
   int f32[4];
 };
 
@@ -859,20 +859,45 @@ int launch_shift_phases(const Dims &D, const double *y, double dgx, double dgz, 
   return hipGetLastError();
 }
 
-// XR holds the eliminated rows in the precision of the array (an FP32 array's
-// D0 solution is kept in FP32: it carries no more digits than the array).
+// The D0 solve's eliminated rows are not stored: the forward sweep keeps the
+// recurrence's state (x1, x2) entering every EXS_CHUNK-th row in CK, and the
+// back substitution, which runs from the top row down, recomputes each chunk
+// of rows from its checkpoint and q before it re-weights them (a chunk's rows
+// of q are all read before its re-weighting writes, and the re-weighting of
+// row i writes row i + 2 of q, above every row a lower chunk reads).  Per
+// array q is read twice and written once: 3/4 of the streamed workspace's
+// bytes.  Alone the kernel is slower than with the workspace (it waits on each
+// chunk's loads), but a time step runs it beside the fused shift and solve on
+// a second stream, where its fewer bytes count: chunks of 8 rows measured
+// fastest for the whole step (16 and 32, and loading the next chunk ahead,
+// slower).
+constexpr int EXS_CHUNK = 8;
 template <typename QT>
 __device__ __forceinline__ void shift_column(const Dims &D, const D0Const &fc, QT *__restrict__ q,
-                                             QT *__restrict__ XR, int col, const double *__restrict__ f0,
+                                             double4 *__restrict__ CK, int col, const double *__restrict__ f0,
                                              const double *__restrict__ der, const double2 *__restrict__ PX,
                                              const double2 *__restrict__ PZ, double sx0s, double sz0s, double sx1s,
                                              double sz1s) {
   int ix, iz;
   col_ix_iz(D, col, ix, iz);
   const size_t nc = D.ncol;
-  const int m = D.ny;
-  C x1, x2, sx0, sx1, r25, r27;
-  d0_forward<0, QT, QT>(D, f0, der, q, XR, col, x1, x2, sx0, sx1, r25, r27);
+  const int m = D.ny, mi = m - 2;
+  constexpr int CH = EXS_CHUNK;
+  C x1{0, 0}, x2{0, 0}, sx0{0, 0}, sx1{0, 0};
+  rows_ascending<PF_FWD, C>(
+      mi, [&](int i) { return ld(q + (size_t)(i + 2) * nc + col); },
+      [&](int i, const C &v) {
+        if (i % CH == 0) CK[(size_t)(i / CH) * nc + col] = make_double4(x1.x, x1.y, x2.x, x2.y);
+        C bx = v;
+        if (i >= 2) bx = bx - F0(i, 0) * x2;
+        if (i >= 1) bx = bx - F0(i, 1) * x1;
+        bx = bx * F0(i, 2);
+        sx0 = sx0 + F0(i, 7) * bx;
+        sx1 = sx1 + F0(i, 8) * bx;
+        x2 = x1;
+        x1 = bx;
+      });
+  const C r25 = ld(q + (size_t)(m - 2 + 2) * nc + col), r27 = ld(q + (size_t)(m - 1 + 2) * nc + col);
   const C ph0 = cexpi(-(D.alfa0 * ix * sx0s + D.beta0 * iz * sz0s));
   C b1, b2;
   d0_border(fc, ph0, x1, x2, sx0, sx1, r25, r27, b1, b2);
@@ -904,7 +929,31 @@ __device__ __forceinline__ void shift_column(const Dims &D, const D0Const &fc, Q
         st(q + (size_t)(r + 2) * nc + col, acc);
         W3 = W2; W2 = W1; W1 = W0; W0 = xp;
       };
-  rows_descending<PF_BWD, C>(m - 2, [&](int i) { return ld(XR + (size_t)i * nc + col); }, body);
+  for (int c = (mi - 1) / CH; c >= 0; c--) {
+    const int c0 = c * CH, n = min(CH, mi - c0);
+    C xr[CH];
+#pragma unroll
+    for (int j = 0; j < CH; j++)
+      if (j < n) xr[j] = ld(q + (size_t)(c0 + j + 2) * nc + col);
+    const double4 ck = CK[(size_t)c * nc + col];
+    C y1{ck.x, ck.y}, y2{ck.z, ck.w};
+#pragma unroll
+    for (int j = 0; j < CH; j++) {
+      if (j < n) {
+        const int i = c0 + j;
+        C bx = xr[j];
+        if (i >= 2) bx = bx - F0(i, 0) * y2;
+        if (i >= 1) bx = bx - F0(i, 1) * y1;
+        bx = bx * F0(i, 2);
+        xr[j] = bx;
+        y2 = y1;
+        y1 = bx;
+      }
+    }
+#pragma unroll
+    for (int j = CH - 1; j >= 0; j--)
+      if (j < n) body(c0 + j, xr[j]);
+  }
   // W = x'(0..3); ghosts below: x'(-1) = x'(ny-1) conj(ph1), x'(-2) = x'(ny-2) conj(ph1)
   const C g1 = xpm1 * conj(ph1), g2 = xpm2 * conj(ph1);
   C acc{0, 0};
@@ -930,11 +979,12 @@ __global__ void __launch_bounds__(256)
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= D.ncol) return;
   const int k = blockIdx.y;
+  // the checkpoints of array k in its XR buffer ((ny-2)/EXS_CHUNK + 1 rows of 32 B)
+  double4 *ck = (double4 *)ql.xr[k];
   if (ql.f32[k])
-    shift_column<float2>(D, fc, (float2 *)ql.q[k], (float2 *)ql.xr[k], col, f0, der, PX, PZ, sx0s, sz0s, sx1s,
-                         sz1s);
+    shift_column<float2>(D, fc, (float2 *)ql.q[k], ck, col, f0, der, PX, PZ, sx0s, sz0s, sx1s, sz1s);
   else
-    shift_column<double2>(D, fc, (double2 *)ql.q[k], ql.xr[k], col, f0, der, PX, PZ, sx0s, sz0s, sx1s, sz1s);
+    shift_column<double2>(D, fc, (double2 *)ql.q[k], ck, col, f0, der, PX, PZ, sx0s, sz0s, sx1s, sz1s);
 }
 
 int launch_exact_shift(const Dims &D, int nq, void *const *q, const int *f32, double2 *const *XR,
