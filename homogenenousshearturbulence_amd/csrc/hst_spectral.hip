@@ -865,12 +865,14 @@ int launch_shift_phases(const Dims &D, const double *y, double dgx, double dgz, 
 // of rows from its checkpoint and q before it re-weights them (a chunk's rows
 // of q are all read before its re-weighting writes, and the re-weighting of
 // row i writes row i + 2 of q, above every row a lower chunk reads).  Per
-// array q is read twice and written once: 3/4 of the streamed workspace's
-// bytes.  Alone the kernel is slower than with the workspace (it waits on each
-// chunk's loads), but a time step runs it beside the fused shift and solve on
-// a second stream, where its fewer bytes count: chunks of 8 rows measured
-// fastest for the whole step (16 and 32, and loading the next chunk ahead,
-// slower).
+// array q is read twice and written once and the FP64 checkpoints (32 B per
+// chunk) written and read: per row 56 B for an FP64 array against 64 B with
+// the streamed workspace, and for an FP32 array (the mixed form's oldrhs) the
+// same 32 B as its FP32 workspace.  Alone the kernel is slower than the
+// streamed one (it waits on each chunk's loads); beside the fused shift and
+// solve of a time step, chunks of 8 rows measured fastest for the whole step
+// (FP64 +1.7 %, mixed unchanged; 16 and 32 rows, and loading the next chunk
+// ahead, slower).
 constexpr int EXS_CHUNK = 8;
 template <typename QT>
 __device__ __forceinline__ void shift_column(const Dims &D, const D0Const &fc, QT *__restrict__ q,
