@@ -482,6 +482,16 @@ struct PentaFactor {
     u21 = u11; u22 = u12; p2 = p1; q2 = q1;
     u11 = t1; u12 = t2; p1 = p; q1 = q;
   }
+  // the state that step() carries from row to row into the matrix rows
+  // (t1, t2, p, q): the last two rows' upper entries and border columns
+  __device__ __forceinline__ void save(double4 *a, double4 *b) const {
+    *a = make_double4(u11, u12, p1, q1);
+    *b = make_double4(u21, u22, p2, q2);
+  }
+  __device__ __forceinline__ void restore(const double4 &a, const double4 &b) {
+    u11 = a.x; u12 = a.y; p1 = a.z; q1 = a.w;
+    u21 = b.x; u22 = b.y; p2 = b.z; q2 = b.w;
+  }
 };
 
 __device__ __forceinline__ int penta_group(const Dims &D, int ix, int iz) { return ix * (D.nz + 1) + (iz < 0 ? -iz : iz); }
@@ -507,9 +517,14 @@ __device__ __forceinline__ bool pair_lane(const Dims &D, int L, int &ix, int &iz
 
 static int pair_lanes(const Dims &D) { return (D.NXm * (D.nz + 1) + 31) / 32 * 64; }
 
-// The matrix part of every (ix, |iz|) group for one kind and lambda:
-// tab[i][g] = (t1, t2, p, q) of row i.  Columns (ix, iz) and (ix, -iz) share
-// k^2 and so every bit of these rows.
+// The matrix part of the back substitution for every (ix, |iz|) group, one
+// kind and lambda: the rows (t1, t2, p, q) come from step() of the matrix
+// factorisation, whose state entering every PC-th row is kept here,
+// tab[2 c + h][g] for chunk c (h = 0: u11, u12, p1, q1; 1: u21, u22, p2, q2);
+// a line recomputes a chunk's rows from it when its back substitution reaches
+// the chunk.  Columns (ix, iz) and (ix, -iz) share k^2 and so every bit of
+// these rows.
+constexpr int PC = 8;
 template <int K>
 __global__ void __launch_bounds__(256) k_penta_factor(Dims D, double lam, const double *__restrict__ der,
                                                       double4 *__restrict__ tab) {
@@ -521,9 +536,47 @@ __global__ void __launch_bounds__(256) k_penta_factor(Dims D, double lam, const 
   const double kk = al * al + be * be;
   PentaFactor f;
   for (int i = 0; i < D.ny - 2; i++) {
+    if (i % PC == 0) f.save(tab + (size_t)(2 * (i / PC)) * ng + g, tab + (size_t)(2 * (i / PC) + 1) * ng + g);
     double l2, l1, rp, t1, t2, p, q;
     f.step<K>(i, lam, D.ni, kk, der, l2, l1, rp, t1, t2, p, q);
-    tab[(size_t)i * ng + g] = make_double4(t1, t2, p, q);
+  }
+}
+
+// The back substitution of one line from its top row down (penta_backward,
+// src/hst_linsolve.f90:359-483, one slab) with the matrix rows recomputed
+// chunk by chunk from the group's checkpoints: x(i) = xr(i) - p zb1 - q zb2 -
+// t1 x(i+1) - t2 x(i+2), xr(i) = loadx(i), each x(i) to emit(i, x).  x1, x2
+// enter as x(mi), x(mi+1) (the border rows).
+template <int K, class LOADX, class EMIT>
+__device__ __forceinline__ void penta_back(const Dims &D, int g, double lam, double kk,
+                                           const double *__restrict__ der, const double4 *__restrict__ tab,
+                                           C zb1, C zb2, C x1, C x2, LOADX loadx, EMIT emit) {
+  const int mi = D.ny - 2, ng = (D.nx + 1) * (D.nz + 1);
+  for (int c = (mi - 1) / PC; c >= 0; c--) {
+    const int c0 = c * PC, n = min(PC, mi - c0);
+    C xr[PC];
+#pragma unroll
+    for (int j = 0; j < PC; j++)
+      if (j < n) xr[j] = loadx(c0 + j);
+    PentaFactor f;
+    f.restore(tab[(size_t)(2 * c) * ng + g], tab[(size_t)(2 * c + 1) * ng + g]);
+    double t1[PC], t2[PC], pp[PC], qq[PC];
+#pragma unroll
+    for (int j = 0; j < PC; j++) {
+      if (j < n) {
+        double l2, l1, rp;
+        f.step<K>(c0 + j, lam, D.ni, kk, der, l2, l1, rp, t1[j], t2[j], pp[j], qq[j]);
+      }
+    }
+#pragma unroll
+    for (int j = PC - 1; j >= 0; j--) {
+      if (j < n) {
+        const C x = xr[j] - pp[j] * zb1 - qq[j] * zb2 - t1[j] * x1 - t2[j] * x2;
+        emit(c0 + j, x);
+        x2 = x1;
+        x1 = x;
+      }
+    }
   }
 }
 
@@ -584,25 +637,25 @@ __device__ __forceinline__ void penta_line(const Dims &D, int col, int ix, int i
   st(dst + (size_t)(m - 1 + 2) * nc + col, b2);
   const C cph = conj(ph);
   const C yb1 = cph * b1, yb2 = cph * b2;
-  C xk1 = b1, xk2 = b2;
-  const int ng = (D.nx + 1) * (D.nz + 1);
-  rows_descending<PF_BWD, WSRow>(
-      mi,
-      [&](int i) {
-        const size_t o = (size_t)i * nc + col;
-        if constexpr (TAB) {
-          const double4 t = tab[(size_t)i * ng + g];
-          return WSRow{ld(ws.XR + o), t.z, t.w, t.x, t.y};
-        } else {
+  if constexpr (TAB) {
+    penta_back<K>(
+        D, g, lam, kk, der, tab, yb1, yb2, b1, b2, [&](int i) { return ld(ws.XR + (size_t)i * nc + col); },
+        [&](int i, const C &x) { st(dst + (size_t)(i + 2) * nc + col, x); });
+  } else {
+    C xk1 = b1, xk2 = b2;
+    rows_descending<PF_BWD, WSRow>(
+        mi,
+        [&](int i) {
+          const size_t o = (size_t)i * nc + col;
           return WSRow{ld(ws.XR + o), ws.B1[o], ws.B2[o], ws.U1[o], ws.U2[o]};
-        }
-      },
-      [&](int i, const WSRow &w) {
-        const C xk = w.xr - w.b1 * yb1 - w.b2 * yb2 - w.u1 * xk1 - w.u2 * xk2;
-        st(dst + (size_t)(i + 2) * nc + col, xk);
-        xk2 = xk1;
-        xk1 = xk;
-      });
+        },
+        [&](int i, const WSRow &w) {
+          const C xk = w.xr - w.b1 * yb1 - w.b2 * yb2 - w.u1 * xk1 - w.u2 * xk2;
+          st(dst + (size_t)(i + 2) * nc + col, xk);
+          xk2 = xk1;
+          xk1 = xk;
+        });
+  }
 }
 
 // blockIdx.y selects the system: K0 for the first, K1 for the second
@@ -627,7 +680,9 @@ __global__ void __launch_bounds__(256)
   else penta_line<K1, TAB>(D, col, ix, iz, g, lam, der, pa.src[1], pa.dst[1], pa.ws[1], pa.tab[1], sx, sz);
 }
 
-size_t penta_table_bytes(const Dims &D) { return (size_t)(D.ny - 2) * (D.nx + 1) * (D.nz + 1) * sizeof(double4); }
+size_t penta_table_bytes(const Dims &D) {
+  return (size_t)2 * ((D.ny - 2 + PC - 1) / PC) * (D.nx + 1) * (D.nz + 1) * sizeof(double4);
+}
 
 int launch_penta_factor(const Dims &D, int kind, double lambda, const double *der, void *tab, hipStream_t s) {
   const int ng = (D.nx + 1) * (D.nz + 1);
@@ -1137,20 +1192,9 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, int ix,
   st(dst + (size_t)(m - 1 + 2) * nc + col, c2);
   const C cph1 = conj(ph1);
   const C zb1 = cph1 * c1, zb2 = cph1 * c2;
-  C zk1 = c1, zk2 = c2;
-  const int ng = (D.nx + 1) * (D.nz + 1);
-  rows_descending<SP_PF, WSRow>(
-      mi,
-      [&](int i) {
-        const double4 t = tab[(size_t)i * ng + g];
-        return WSRow{ld(XRp + (size_t)i * nc + col), t.z, t.w, t.x, t.y};
-      },
-      [&](int i, const WSRow &w) {
-        const C zk = w.xr - w.b1 * zb1 - w.b2 * zb2 - w.u1 * zk1 - w.u2 * zk2;
-        st(dst + (size_t)(i + 2) * nc + col, zk);
-        zk2 = zk1;
-        zk1 = zk;
-      });
+  penta_back<K>(
+      D, g, lam, kk, der, tab, zb1, zb2, c1, c2, [&](int i) { return ld(XRp + (size_t)i * nc + col); },
+      [&](int i, const C &x) { st(dst + (size_t)(i + 2) * nc + col, x); });
 }
 
 struct ShiftPentaArgs {
