@@ -103,8 +103,9 @@ int launch_buildrhs(const Dims &D, int mixed, const RhsArgs &a, const double2 *V
 int launch_penta(const Dims &D, int nsys, const int *kinds, double lambda, const double2 *const *src,
                  double2 *const *dst, const PentaWS *ws, const void *const *tabs, const double *der, double sx,
                  double sz, hipStream_t s);
-// The matrix part of the D2V or ETA lines for lambda, one row set per
-// (ix, |iz|) group (penta_table_bytes of device memory).
+// The matrix part of the D2V or ETA lines for lambda: per (ix, |iz|) group
+// the factorisation's state every 8 rows, from which the solves recompute
+// the rows (penta_table_bytes of device memory).
 size_t penta_table_bytes(const Dims &D);
 int launch_penta_factor(const Dims &D, int kind, double lambda, const double *der, void *tab, hipStream_t s);
 
