@@ -1127,29 +1127,24 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, int ix,
   const double kk = al * al + be * be, ni = D.ni;
   PentaFactor f;
   C px1{0, 0}, px2{0, 0}, psx0{0, 0}, psx1{0, 0}, r25p{0, 0}, r27p{0, 0};
+  // row i (< mi) of the elimination
   auto feed = [&](int i, const C &bin) {
-    if (i < mi) {
-      double l2, l1, rp, t1, t2, p, qq;
-      f.step<K>(i, lam, ni, kk, der, l2, l1, rp, t1, t2, p, qq);
-      C bx = bin;
-      if (i >= 2) bx = bx - l2 * px2;
-      if (i >= 1) bx = bx - l1 * px1;
-      bx = bx * rp;
-      st(XRp + (size_t)i * nc + col, bx);
-      psx0 = psx0 + f.ww * bx;
-      psx1 = psx1 + f.vv * bx;
-      px2 = px1;
-      px1 = bx;
-    } else if (i == mi) {
-      r25p = bin;
-    } else {
-      r27p = bin;
-    }
+    double l2, l1, rp, t1, t2, p, qq;
+    f.step<K>(i, lam, ni, kk, der, l2, l1, rp, t1, t2, p, qq);
+    C bx = bin;
+    if (i >= 2) bx = bx - l2 * px2;
+    if (i >= 1) bx = bx - l1 * px1;
+    bx = bx * rp;
+    st(XRp + (size_t)i * nc + col, bx);
+    psx0 = psx0 + f.ww * bx;
+    psx1 = psx1 + f.vv * bx;
+    px2 = px1;
+    px1 = bx;
   };
   // window W = x'(i-2 .. i+1) for the row i about to be emitted; below row 0
   // the ghosts x'(-k) = x'(m-k) conj(ph1)
   C W0 = xr1 * phase(m - 2) * conj(ph1), W1 = xr0 * phase(m - 1) * conj(ph1), W2 = xp0, W3 = xp1;
-  auto emit = [&](int i, const C &xnext) {  // xnext = x'(i + 2)
+  auto reweight = [&](int i, const C &xnext) {  // row i of D0 x'; xnext = x'(i + 2)
     C acc{0, 0};
     acc = acc + DER(der, i, 0, -2) * W0;
     acc = acc + DER(der, i, 0, -1) * W1;
@@ -1157,7 +1152,7 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, int ix,
     acc = acc + DER(der, i, 0, 1) * W3;
     acc = acc + DER(der, i, 0, 2) * xnext;
     W0 = W1; W1 = W2; W2 = W3; W3 = xnext;
-    feed(i, acc);
+    return acc;
   };
   struct Row {
     C xr, ph;
@@ -1169,10 +1164,13 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, int ix,
         const C xk = r.xr - F0(k, 5) * yb1 - F0(k, 6) * yb2 - F0(k, 3) * xk1 - F0(k, 4) * xk2;
         xk2 = xk1;
         xk1 = xk;
-        emit(m - 1 - k - 2, xk * r.ph);  // x'(m-1-k) completes row m-3-k
+        const int i = m - 1 - k - 2;  // x'(m-1-k) completes row m-3-k < mi
+        feed(i, reweight(i, xk * r.ph));
       });
-  emit(m - 2, xp0 * ph1);  // x'(m) = x'(0) ph1
-  emit(m - 1, xp1 * ph1);  // x'(m+1) = x'(1) ph1
+  // the border rows' right-hand sides (the loop above covers the rows < mi,
+  // so that its body has no branch on the row)
+  r25p = reweight(m - 2, xp0 * ph1);  // x'(m) = x'(0) ph1
+  r27p = reweight(m - 1, xp1 * ph1);  // x'(m+1) = x'(1) ph1
   // 3. the line solve's border system and back substitution (as penta_line)
   double rq[25];
   rq[3] = f.sp0; rq[4] = f.sq0; rq[5] = f.w2 * f.u22 + f.w1 * f.u11; rq[6] = f.w1 * f.u12;
