@@ -4,6 +4,9 @@
   box128_y257.in): two steps against the oracle, which finishes them in
   seconds on the host's cores, per field at the reference's 1e-10 regression
   bar (FP64) and at 1e-5 (the mixed form).
+* nx = nz = 256, ny = 513 with the exact shift (Re = 8000, dt = 0.001; the
+  metric deck box256_y513_shift.in): one step against the oracle (about 50 s
+  of the host's cores), at 1e-10 (FP64) and 1e-5 (the mixed form).
 * nx = nz = 512, ny = 1025 (nxd = 1024, nzd = 1536), the mixed form, Re =
   20000, dt = 5e-4 (box512_y1025.in), through a property that does not
   depend on the size: steps with the physical-space planes processed in
@@ -48,6 +51,26 @@ def test_box128_steps_match_oracle(box128_oracle, precision, tol):
     finally:
         g.close()
     print("nx = nz = 128, ny = 257, two steps: rel L2 (u, v, w)", errs)
+    assert max(errs) <= tol
+
+
+@pytest.fixture(scope="module")
+def box256_oracle(oracle_built):
+    d = read_input(os.path.join(DECKS, "box256_y513_shift.in"))
+    return d, oracle_built.run_deck(d, nsteps=1)
+
+
+@pytest.mark.parametrize("precision,tol", [(hst.FP64, 1e-10), (hst.MIXED, 1e-5)])
+def test_box256_shift_step_matches_oracle(box256_oracle, precision, tol):
+    d, o = box256_oracle
+    g = hst.run_deck(d, precision, nsteps=1)
+    try:
+        assert g.time == o.time
+        V = g.V
+        errs = [refops.rel_l2(V[c][..., 2:-2], o.V[c][..., 2:-2]) for c in range(3)]
+    finally:
+        g.close()
+    print("nx = nz = 256, ny = 513, exact shift, one step: rel L2 (u, v, w)", errs)
     assert max(errs) <= tol
 
 
