@@ -409,8 +409,9 @@ __global__ void __launch_bounds__((XCfg<R, M>::NT))
 
 // ------------------------------------------------------------------ k_xline
 
-// The x pass for power-of-two lengths M >= 64: T threads per line (T = M/8,
-// at most 64), E = M/T elements per thread, one wavefront per block.  A
+// The x pass for power-of-two lengths M >= 64: T = M/8 threads per line,
+// E = 8 elements per thread; a block is one wavefront holding 64/T lines, or
+// at M = 1024 two wavefronts holding one line.  A
 // thread holds the elements t + T e (e < E) of a line in registers; the
 // Stockham stages exchange through LDS only between stages, and the last
 // stage of a transform leaves the elements t + T e in the registers of
@@ -419,9 +420,10 @@ __global__ void __launch_bounds__((XCfg<R, M>::NT))
 // the forward transforms.  LDS lines are padded by one element per eight.
 template <int M>
 struct XL {
-  static constexpr int T = (M / 8 < 64) ? M / 8 : 64;
+  static constexpr int T = M / 8;
   static constexpr int E = M / T;
-  static constexpr int LPB = 64 / T;
+  static constexpr int NTH = T < 64 ? 64 : T;  // threads per block
+  static constexpr int LPB = NTH / T;
   static constexpr int PL = M + M / 8;
 };
 
@@ -526,7 +528,7 @@ __device__ __forceinline__ void xl_c2r_load(Cx<R> (&dst)[XL<M>::E], const Cx<R> 
   }
 }
 
-// occupancy floor: the FP64 lines of M <= 512 fit in 256 VGPRs (two
+// occupancy floor (wavefronts per SIMD): the FP64 lines fit in 256 VGPRs (two
 // wavefronts per SIMD) with a few bytes of spill instead of taking one; the
 // FP32 lines are bound by the vector ALU and run 4 % faster at four
 // wavefronts with 36 bytes of spill than at three without
@@ -534,17 +536,17 @@ __device__ __forceinline__ void xl_c2r_load(Cx<R> (&dst)[XL<M>::E], const Cx<R> 
 #define XLINE_F32_WAVES 4
 #endif
 template <typename R, int M>
-constexpr int xline_min_waves() { return sizeof(R) == 8 ? (M >= 1024 ? 1 : 2) : (M >= 1024 ? 2 : XLINE_F32_WAVES); }
+constexpr int xline_min_waves() { return sizeof(R) == 8 ? 2 : XLINE_F32_WAVES; }
 
 template <typename R, int M, int MODE>
-__global__ void __launch_bounds__(64, (xline_min_waves<R, M>()))
+__global__ void __launch_bounds__(XL<M>::NTH, (xline_min_waves<R, M>()))
     k_xline(const Cx<R> *__restrict__ Hv, Cx<R> *__restrict__ Hp, Dims D, int nrows, const Cx<R> *__restrict__ twx,
             const Cx<R> *__restrict__ twh, const double *__restrict__ dyl, double *__restrict__ cfl_part,
             int want_cfl, int row0) {
   constexpr int T = XL<M>::T, E = XL<M>::E, LPB = XL<M>::LPB, PL = XL<M>::PL;
   __shared__ Cx<R> lds[LPB * PL];
   __shared__ Cx<R> twl[xtw_size<M>() > 0 ? xtw_size<M>() : 1];  // the staged twiddles
-  for (int k = threadIdx.x; k < xtw_size<M>(); k += 64) twl[k] = twx[xtw_exponent<M>(k)];
+  for (int k = threadIdx.x; k < xtw_size<M>(); k += blockDim.x) twl[k] = twx[xtw_exponent<M>(k)];
   __syncthreads();
   const int g = threadIdx.x / T, t = threadIdx.x % T;
   const int NZD = D.nzd, nx = D.nx;
@@ -577,7 +579,17 @@ __global__ void __launch_bounds__(64, (xline_min_waves<R, M>()))
       }
     }
     for (int o = 32; o > 0; o >>= 1) cm = fmax(cm, __shfl_xor(cm, o));
-    if (threadIdx.x == 0) cfl_part[blockIdx.x] = cm;
+    if constexpr (XL<M>::NTH > 64) {  // the block's wavefronts
+      __shared__ double red[XL<M>::NTH / 64];
+      if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cm;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        for (int i = 1; i < XL<M>::NTH / 64; i++) cm = fmax(cm, red[i]);
+        cfl_part[blockIdx.x] = cm;
+      }
+    } else {
+      if (threadIdx.x == 0) cfl_part[blockIdx.x] = cm;
+    }
   }
   if constexpr (MODE == 2) return;
   const R factor = (R)D.factor;
@@ -656,7 +668,7 @@ static int run_xpass(const Dims &D, int mode, const void *Hv, void *Hp, int row0
     dim3 grid((lines + C::LPB - 1) / C::LPB);
     if (n_part) *n_part = grid.x;
 #define LX(MD, WC)                                                                                                 \
-  hipLaunchKernelGGL((k_xline<R, M, MD>), grid, dim3(64), 0, s, (const Cx<R> *)Hv, (Cx<R> *)Hp, D, nrows, tb.twx, \
+  hipLaunchKernelGGL((k_xline<R, M, MD>), grid, dim3(C::NTH), 0, s, (const Cx<R> *)Hv, (Cx<R> *)Hp, D, nrows, tb.twx, \
                      tb.twh, dyl, cfl_part, WC, row0)
     if (mode == 0) LX(0, want_cfl);
     else if (mode == 1) LX(1, want_cfl);

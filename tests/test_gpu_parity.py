@@ -133,6 +133,33 @@ def test_products_power_of_two_lengths(O, nx, precision, tol):
             assert rel(g.product(p + 1), P0[p]) <= tol, p
 
 
+@pytest.mark.parametrize("nx", [15, 42, 682])
+@pytest.mark.parametrize("precision,tol", [(hst.FP64, 1e-12), (hst.MIXED, 1e-5)])
+def test_cfl_matches_oracle(O, nx, precision, tol):
+    """compute_cfl (src/hst_transforms.f90:163-179): the maximum over the
+    interior rows and the physical points of |u|/dx + |v|/dyl + |w|/dz, through
+    the generic x pass (nxd = 24), a wavefront holding eight x lines (nxd = 64)
+    and two wavefronts per x line (nxd = 1024), vs the oracle; and the number a
+    time step reports from its third substep."""
+    d = read_input(deck_path("small"))
+    if nx != d.nx:
+        d = dataclasses.replace(d, nx=nx, nz=4, ny=12, kpeak=3.0)
+    o = seeded(O, d, 0.17)
+    want = o.nonlinear(mode=0, want_cfl=True)
+    with GPU(d, precision) as g:
+        g.set_V(o.V)
+        g.time = o.time
+        got = [g.nonlinear(want_cfl=True), g.compute_cfl()]
+        o.timestep(d.deltat)
+        g.timestep(d.deltat, want_cfl=True)
+        got_step = g.cfl
+    print(nx, precision, "cfl", want, got, "after a step", o.cfl, got_step)
+    assert want > 0 and o.cfl > 0
+    for c in got:
+        assert abs(c - want) <= tol * want
+    assert abs(got_step - o.cfl) <= max(tol, 1e-10) * o.cfl
+
+
 @pytest.mark.parametrize("nz", [16, 32, 64, 128, 256])
 @pytest.mark.parametrize("precision,tol", [(hst.FP64, 1e-12), (hst.MIXED, 2e-5)])
 def test_products_z_lengths(O, nz, precision, tol):
