@@ -245,3 +245,37 @@ def test_attached_step_matches_oracle(O, filtered):
         print(k, "attached vs oracle", e, "attached vs plain", d)
         assert e < 1e-6
         assert d > 100 * e
+
+
+def test_steps_across_attach_and_detach(filtered):
+    """Between plain time steps the velocity is held as eta and dv/dy; a step
+    with the closure attached (its hook reads u, v, w) restores u, w first,
+    and the plain steps after a detach go back to the compact form.  Plain,
+    attached, attached, plain, plain steps in a row against the same sequence
+    with the velocity read after every step (a restore each time)."""
+    f = filtered
+    cd = f["cd"]
+    nm = norm_of(f["X"].astype(np.float64), f["Y"].astype(np.float64))
+    p = random_params(8, scale=0.2)
+    res = []
+    for read_between in (False, True):
+        s = hst.Solver(cd, hst.FP64)
+        c = closure.Closure(cd, 0)
+        try:
+            c.set_params(p)
+            c.set_norm(nm)
+            s.set_V(f["U"])
+            s.reset_old()
+            s.time = f["t"]
+            for attached in (False, True, True, False, False):
+                c.attach(attached)
+                s.timestep(cd.deltat)
+                if read_between:
+                    assert np.all(np.isfinite(s.V))
+            c.attach(False)
+            res.append(s.V)
+        finally:
+            c.close()
+            s.close()
+    print("attach/detach sequence, with and without reads between", rel(res[0], res[1]))
+    assert rel(res[0], res[1]) < 1e-12
