@@ -1,5 +1,5 @@
 """The learned closure end to end: a DNS generates the data, the network is
-trained on the GPU, and a coarse run with the network is judged against the
+trained on the GPU, and a coarse run with the network is measured against the
 filtered DNS beside the coarse run without it (the plain method).
 
     python -m homogenenousshearturbulence_amd.les [--out result.json] [--quick]
@@ -12,11 +12,11 @@ Steps (no counterpart in the reference; every GPU step is libhstgpu.so):
      the training times the velocity and its six products are
      filtered onto the coarse grid (les32_y65: nx = nz = 32, ny = 65) and the
      (gradients, stresses) of every coarse grid point join the data set; at
-     the record times the filtered velocity is kept as the judge.
+     the record times the filtered velocity is kept as the yardstick.
   2. Training: Adam over the data set, the last snapshot held out.
   3. Coarse runs from the filtered DNS field at t0: without the closure and
      with it attached; the same records.
-  4. The judge: the filtered DNS's energy, dissipation and uv stress histories
+  4. The yardstick: the filtered DNS's energy, dissipation and uv stress histories
      and its x and z spectra against each coarse run.
 
 If the run with the closure stops producing finite statistics it is stopped
@@ -176,7 +176,7 @@ def spectra(V: np.ndarray, deck: Deck):
     return ex, ez
 
 
-def judge(plan: Plan, ref: dict, runs: dict) -> dict:
+def evaluate(plan: Plan, ref: dict, runs: dict) -> dict:
     """Step 4: errors of each coarse run against the filtered DNS."""
     sref = statistics(plan, ref)
     result = {"times": list(plan.record_times), "filtered_dns": {}, "runs": {}}
@@ -217,7 +217,7 @@ def run(plan: Plan, log=print) -> dict:
         model = coarse_run(plan, U0, c, log)
     finally:
         c.close()
-    res = judge(plan, records, {"plain": plain, "closure": model})
+    res = evaluate(plan, records, {"plain": plain, "closure": model})
     res["train_loss"] = [float(v) for v in loss]
     res["heldout_loss"] = [float(v) for v in val]
     res["samples"] = int(plan.les.ny * plan.les.nzd * 2 * plan.les.nxd * len(plan.t_train))

@@ -3,7 +3,7 @@
 The DNS of the 128 x 257 x 128 grid (Re 1000) generates the data; the network is
 trained on the GPU by the library's kernels; two coarse runs on the 32 x 65 x 32
 grid start from the filtered DNS field, one with the closure attached and one
-without it (the plain method); the filtered DNS is the judge.  The measured
+without it (the plain method); the filtered DNS is the yardstick.  The measured
 errors (mean relative error of the resolved energy and dissipation over the
 records, and the mean |log| ratio of the x and z spectra over the upper half of
 the resolved wavenumbers at the last record) must all be smaller with the
