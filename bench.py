@@ -1,5 +1,5 @@
 """RK3 steps/s of the hst substep on one MI355X at nx = nz = 256, ny = 513
-(the metric deck, homogenenousshearturbulence_amd/decks/box256_y513_shift.in: Re 8000,
+(the deck homogenenousshearturbulence_amd/decks/box256_y513_shift.in: Re 8000,
 exact_shift, dt = 0.001): the mixed-precision form (FP32 transforms, products
 and derivative right-hand sides; FP64 solves, Crank-Nicolson operators, mean
 mode and phase) against the FP64 form, the relative L2 error of the mixed
@@ -147,7 +147,7 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+        print(f"bench.py runs on one GPU: this library has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
     deck = read_input(args.deck)

@@ -24,7 +24,7 @@ CONFIGS = {
     # name: (nx=nz, ny, re, kpeak, exact_shift, deltat, nstep, note)
     "box32_y65": (32, 65, 1000.0, 4.0, False, 0.005, 100, "correctness deck: FP64 vs the CPU oracle"),
     "box128_y257": (128, 257, 4000.0, 8.0, False, 0.002, 200, "FP64 on one GPU"),
-    "box256_y513_shift": (256, 513, 8000.0, 12.0, True, 0.001, 100, "the metric deck: mixed vs FP64"),
+    "box256_y513_shift": (256, 513, 8000.0, 12.0, True, 0.001, 100, "the exact-shift deck: mixed vs FP64"),
     "box512_y1025": (512, 1025, 20000.0, 16.0, False, 5e-4, 20, "mixed precision at the largest one-GPU grid"),
     "les32_y65": (32, 65, 1000.0, 8.0, False, 0.002, 400,
                   "coarse run of the learned closure: the grid of box128_y257 filtered by 4, at Re 1000"),

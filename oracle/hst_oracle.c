@@ -1,7 +1,7 @@
 /*
  * hst_oracle.c -- FP64 CPU restatement of hst's RK3 step (one rank, npy = 1).
  *
- * TEST INFRASTRUCTURE ONLY.  This is the checker the GPU library is compared
+ * TEST INFRASTRUCTURE ONLY.  This is the reference the GPU library is compared
  * with; only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
  * load it.  The product (libhstgpu.so) never calls it.
  *
@@ -1324,7 +1324,8 @@ int hsto_max_threads(void) {
  *
  * The learned subgrid closure has no counterpart in the reference: these
  * functions restate this library's own definitions (the filter, the samples,
- * the network and its gradient) so that the GPU kernels have a checker.
+ * the network and its gradient) so that the GPU kernels have a reference to
+ * be compared with.
  *
  * Pinning:
  *   filter weights     closed forms: unit sum, the 4:1 trapezoid [1/8 1/4 1/4 1/4 1/8],

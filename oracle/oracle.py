@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/libhst_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py, as the checker.  The product never imports it.
+CPU-baseline leg of bench.py, as the reference the GPU library is tested
+against.  The product never imports it.
 """
 from __future__ import annotations
 

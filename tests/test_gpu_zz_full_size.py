@@ -5,7 +5,7 @@
   seconds on the host's cores, per field at the reference's 1e-10 regression
   bar (FP64) and at 1e-5 (the mixed form).
 * nx = nz = 256, ny = 513 with the exact shift (Re = 8000, dt = 0.001; the
-  metric deck box256_y513_shift.in): one step against the oracle (about 50 s
+  deck box256_y513_shift.in): one step against the oracle (about 50 s
   of the host's cores), at 1e-10 (FP64) and 1e-5 (the mixed form).
 * nx = nz = 512, ny = 1025 (nxd = 1024, nzd = 1536), the mixed form, Re =
   20000, dt = 5e-4 (box512_y1025.in), through a property that does not

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (the checker of the GPU library) against what the
+"""Pins the CPU oracle (what the GPU library is compared with) against what the
 reference itself fixes: its 50-step golden fields, its closed-form tests, and
 brute force / mathematics where those leave something open."""
 import dataclasses
