@@ -75,54 +75,83 @@ def run_form(deck, precision, V0, warmup, steps, acc_steps, prof_steps=2, chunk=
         s.close()
 
 
-def cpu_baseline(deck, threads):
-    """The FP64 CPU oracle on the host: one full substep (transforms, products,
-    right-hand sides, exact shift, solves) of the same deck, scaled to steps/s."""
+def nproc() -> int:
+    """The host threads this process may use, as nproc(1) counts them: the CPU
+    affinity mask, capped by OMP_NUM_THREADS and OMP_THREAD_LIMIT."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    for var in ("OMP_NUM_THREADS", "OMP_THREAD_LIMIT"):
+        try:
+            n = min(n, int(os.environ[var].split(",")[0]))
+        except (KeyError, ValueError):
+            pass
+    return max(n, 1)
+
+
+def cpu_step(deck, threads):
+    """The FP64 CPU oracle on the host: warm-up = the start-up transform and
+    CFL number of the seeded field (every array touched once), then one whole
+    RK3 step (three substeps: all three rows of the RK3 table, the transforms,
+    products, right-hand sides, shift and solves) timed with a monotonic clock.
+    Returns (seconds per step, warm-up seconds)."""
     from oracle import oracle
 
     oracle.set_threads(threads)
     o = oracle.Oracle(deck)
     o.initial_field()
     o.fill_all_ghosts(0.0)
-    import ctypes
-
-    L = oracle.lib()
-    ode = (ctypes.c_double * 3)(*oracle.RK_RAI[0])
-    dt = deck.deltat
     t0 = time.perf_counter()
-    o.nonlinear(mode=0)
-    L.hsto_buildrhs_prepare(o.h, ode, dt, 0.0)
-    L.hsto_buildrhs(o.h, ode, 1)
-    L.hsto_buildrhs(o.h, ode, 2)
-    dts = 2.0 / oracle.RK_RAI[0][0] * dt
-    L.hsto_shear_shift(o.h, dts, 0.0)
-    L.hsto_linsolve(o.h, oracle.RK_RAI[0][0] / dt, dts)
-    sub = time.perf_counter() - t0
+    oracle.compute_cfl(o)
+    warm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.timestep(deck.deltat)
+    step = time.perf_counter() - t0
     del o
-    return {"value": 1.0 / (3.0 * sub), "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"one RK3 substep of box256_y513_shift (nx=nz=256, ny=513) timed in {sub:.1f} s with {threads} threads, "
-                      "steps/s = 1/(3 x substep time)"}
+    return step, warm
 
 
-def copy_probe(mib):
-    """Five launches of a plain elementwise copy (read n bytes, write n bytes):
-    what FETCH_SIZE and WRITE_SIZE read for it scales the counters of the
-    solver's kernels to bytes."""
-    import torch
+def cpu_baseline(deck, threads, small_deck):
+    """BASELINE.md's CPU plan with the build's FP64 oracle: one whole step of
+    the metric deck at nproc threads (the value), and one whole step of the
+    nx = nz = 32, ny = 65 deck (the CPU-runnable correctness workload) at 1
+    thread and at nproc threads: one thread on the metric deck (about 16 times
+    the nproc time) or on nx = nz = 128, ny = 257 (over two minutes a step) does
+    not fit the benchmark's time limit."""
+    step, warm = cpu_step(deck, threads)
+    log(f"CPU oracle, {threads} threads: {step:.1f} s per step (warm-up {warm:.1f} s)")
+    s1, _ = cpu_step(small_deck, 1)
+    log(f"CPU oracle, box32_y65, 1 thread: {s1:.3f} s per step")
+    sn, _ = cpu_step(small_deck, threads)
+    log(f"CPU oracle, box32_y65, {threads} threads: {sn:.3f} s per step")
+    return {"value": 1.0 / step, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"one whole RK3 step of box256_y513_shift (nx=nz=256, ny=513, exact_shift) after a warm-up "
+                      f"transform ({warm:.1f} s): {step:.1f} s with {threads} threads (nproc)",
+            "box32_y65": {"workload": "nx=nz=32, ny=65, Re=1000, dt=0.005 (one whole RK3 step after a warm-up "
+                                      "transform)",
+                          "one_thread": {"value": round(1.0 / s1, 4), "unit": "steps/s", "s_per_step": round(s1, 4)},
+                          "nproc": {"value": round(1.0 / sn, 4), "unit": "steps/s", "s_per_step": round(sn, 4),
+                                    "cores": threads}}}
 
-    n = mib * (1 << 20) // 4
-    x = torch.rand(n, device="cuda:0", dtype=torch.float32)
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record()
-    for _ in range(5):
-        torch.mul(x, 1.0, out=y)
-    b.record()
-    torch.cuda.synchronize()
-    ms = a.elapsed_time(b) / 5
-    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 5,
-                      "ms_per_launch": ms, "GBps": 8 * n / (ms * 1e-3) / 1e9}), flush=True)
+
+def copy_probe(gib):
+    """The library's bandwidth probe on `gib` GiB buffers, one timed launch per
+    grid (read-only, write-only, copy kernels): what FETCH_SIZE and WRITE_SIZE
+    read for its copy kernel (gib GiB read, gib GiB written per launch) scales
+    the counters of the solver's kernels to bytes."""
+    print(json.dumps({"bandwidth_probe": hst.bandwidth_probe(gib, 1)}), flush=True)
+
+
+def kernel_sources_sha() -> str:
+    """sha256 of the HIP sources and headers of the library: the traffic
+    figures of profiles/traffic.json hold for the build they were measured on."""
+    import hashlib
+
+    csrc = os.path.join(ROOT, "homogenenousshearturbulence_amd", "csrc")
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(csrc)):
+        if name.endswith((".hip", ".h")):
+            with open(os.path.join(csrc, name), "rb") as fh:
+                h.update(name.encode() + b"\0" + fh.read())
+    return h.hexdigest()
 
 
 def step_bytes(table, steps):
@@ -140,8 +169,8 @@ def main():
     ap.add_argument("--deck", default=DECK)
     ap.add_argument("--chunk", type=int, default=0, help="rows per transform chunk (0: all rows)")
     ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="only run five plain device copies of MIB MiB (the HBM counter calibration) and exit")
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
+                    help="only run the bandwidth probe once on GIB GiB buffers (the HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -165,6 +194,9 @@ def main():
                           "kernels": {x["kernel"]: round(x["ms"] / x["launches"], 4) for x in tab},
                           "kernel_ms_per_step": {x["kernel"]: round(x["ms"] / 2, 3) for x in tab}}), flush=True)
         return
+    log("bandwidth probe (4 GiB streaming kernels)")
+    probe = hst.bandwidth_probe(4.0, 5)
+    log(f"HBM measured: read {probe['read_GBps']} GB/s, write {probe['write_GBps']} GB/s, copy {probe['copy_GBps']} GB/s")
     log("FP64 form")
     r64 = run_form(deck, hst.FP64, V0, args.warmup, args.steps, acc, chunk=args.chunk)
     log(f"FP64: {args.steps / r64['elapsed']:.3f} steps/s")
@@ -193,13 +225,20 @@ def main():
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     # HBM bytes per launch of that kernel from the FETCH_SIZE / WRITE_SIZE runs
     # (profiles/traffic_box256_y513_shift_mixed.md), scaled by the copy calibration
+    # (profiles/traffic_*.md), scaled by the copy calibration; null when the
+    # kernels changed since (the sources' hash differs)
     traffic = None
+    traffic_note = "no counter run for this workload"
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             t = json.load(fh)
         if t.get("workload") == os.path.basename(args.deck)[:-3] and t.get("form") == "mixed":
-            traffic = t.get("bytes_per_launch", {}).get(dom["kernel"])
+            if t.get("sources_sha256") == kernel_sources_sha():
+                traffic = t.get("bytes_per_launch", {}).get(dom["kernel"])
+                traffic_note = "rocprofv3 FETCH_SIZE/WRITE_SIZE of this build (profiles/traffic_*.md)"
+            else:
+                traffic_note = "the counter run in profiles/ is of other kernel sources: not used"
     bytes_mx = step_bytes(rmx["table"], 2)
     bytes_64 = step_bytes(r64["table"], 2)
     kernels = {r["kernel"]: {"ms_per_launch": round(r["ms"] / r["launches"], 4),
@@ -211,9 +250,10 @@ def main():
                  for r in k64}
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or nproc()
         log(f"CPU oracle baseline, {threads} threads")
-        cpu = cpu_baseline(deck, threads)
+        small = read_input(os.path.join(ROOT, "homogenenousshearturbulence_amd", "decks", "box32_y65.in"))
+        cpu = cpu_baseline(deck, threads, small)
         log(f"CPU: {cpu['value']:.4f} steps/s")
     out = {
         "metric": METRIC,
@@ -247,7 +287,9 @@ def main():
         },
         "roofline": {"kernel": dom["kernel"], "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic},
+                     "traffic": traffic, "traffic_source": traffic_note,
+                     "measured_ceiling": probe,
+                     "frac_of_measured_copy": round(achieved / probe["copy_GBps"], 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -59,6 +59,8 @@ struct State {
   D0Const fcr;
   double *d_f0r = nullptr;
   size_t part_cap = 0;
+  double *h_part = nullptr;  // pinned host copy of d_part (the CFL maxima of a time step)
+  int n_part_pending = 0;
   double2 *F[9] = {nullptr};
   void *P = nullptr, *Hv = nullptr, *Hp = nullptr, *ws = nullptr;
   size_t ws_bytes = 0;
@@ -311,7 +313,10 @@ static const void *P_of(int p) {  // product p = 0..5
 
 // mode 0: the six products; 1: the round trip; 2: the CFL number only;
 // 3: the five combinations vv - uu, vv - ww, uv, vw, uw (a time step's own)
-static int nonlinear_pass(int mode, bool want_cfl, double *cfl) {
+// defer: the CFL partial maxima are copied to host memory on the stream and
+// reduced by cfl_finish() after the caller's next synchronisation, so that the
+// rest of a time step is queued without a wait.
+static int nonlinear_pass(int mode, bool want_cfl, double *cfl, bool defer = false) {
   const double2 *V[3] = {S.F[1], S.F[2], S.F[3]};
   const int np = (mode == 0 ? 6 : (mode == 3 ? 5 : 3));
   int npart_total = 0;
@@ -334,7 +339,10 @@ static int nonlinear_pass(int mode, bool want_cfl, double *cfl) {
     });
     if (rc) return rc;
   }
-  if (want_cfl && cfl) {
+  if (want_cfl && cfl && defer) {
+    HIPCHK(hipMemcpyAsync(S.h_part, S.d_part, sizeof(double) * npart_total, hipMemcpyDeviceToHost, S.stream));
+    S.n_part_pending = npart_total;
+  } else if (want_cfl && cfl) {
     std::vector<double> h(npart_total);
     HIPCHK(hipMemcpyAsync(h.data(), S.d_part, sizeof(double) * npart_total, hipMemcpyDeviceToHost, S.stream));
     HIPCHK(hipStreamSynchronize(S.stream));
@@ -342,6 +350,16 @@ static int nonlinear_pass(int mode, bool want_cfl, double *cfl) {
     for (double v : h) c = std::max(c, v);
     *cfl = c;
   }
+  return 0;
+}
+
+// the CFL number of a deferred nonlinear_pass (the stream synchronised here)
+static int cfl_finish(double *cfl) {
+  HIPCHK(hipStreamSynchronize(S.stream));
+  double c = 0.0;
+  for (int k = 0; k < S.n_part_pending; k++) c = std::max(c, S.h_part[k]);
+  S.n_part_pending = 0;
+  *cfl = c;
   return 0;
 }
 
@@ -623,7 +641,11 @@ static void release() {
   for (void *p : {(void *)S.d_gdamp, (void *)S.d_fy, (void *)S.d_inlayer})
     if (p) (void)hipFree(p);
   if (S.PZ) (void)hipFree(S.PZ);
+  if (S.h_part) (void)hipHostFree(S.h_part);
   S = State();
+  // an attached closure belongs to the state released here: the next state
+  // (perhaps on another grid) steps without it until it is attached again
+  set_stress_hook(nullptr, nullptr);
 }
 
 }  // namespace hst
@@ -685,6 +707,10 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
   HIPCHK(hipEventCreateWithFlags(&S.ev_fork, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&S.ev_join, hipEventDisableTiming));
   if (const char *e = std::getenv("HST_SERIAL")) S.serial = std::atoi(e) != 0;
+  // HST_NO_TABLES=1: no factor tables (the path taken when they do not fit in
+  // device memory): the exact shift of rhs runs apart from the line solves,
+  // which eliminate their rows through the full workspace
+  if (const char *e = std::getenv("HST_NO_TABLES")) S.ptab_off = std::atoi(e) != 0;
   const size_t nf = (size_t)D.NYg * D.ncol;
   const size_t cs = S.mixed ? sizeof(float2) : sizeof(double2);
   int64_t held = 0;
@@ -766,6 +792,7 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
   HIPCHK(hipMemcpy(S.d_f0r, S.f0r.rows.data(), S.f0r.rows.size() * sizeof(double), hipMemcpyHostToDevice));
   S.part_cap = std::max<size_t>((size_t)D.NYg * D.nzd, (size_t)((D.ncol + 255) / 256) * NSTAT + 64);
   HIPCHK(hipMalloc(&S.d_part, S.part_cap * sizeof(double)));
+  HIPCHK(hipHostMalloc((void **)&S.h_part, S.part_cap * sizeof(double), hipHostMallocDefault));
   if (make_fft_tables(D, S.mixed, &S.tables)) return fail("twiddle tables");
   S.bytes_held = held;
   S.live = true;
@@ -936,6 +963,10 @@ int hst_gpu_linsolve(double lambda, double time) {
 int hst_gpu_timestep(double deltat, double *time, double *cfl) {
   if (int rc = check_live_nojoin()) return rc;
   if (!time) return fail("time is NULL");
+  // the substep divides by deltat (lambda = RK(i,0)/deltat): a deck with
+  // deltat = 0 takes its step from the CFL number (new_timestep) first
+  if (!(std::isfinite(deltat) && deltat > 0.0))
+    return fail("timestep: deltat must be finite and > 0 (got " + std::to_string(deltat) + ")");
   double t = *time, c = 0.0;
   // between substeps (and calls) eta and dv/dy stand for u and w unless a
   // hook reads the velocity (closure, Stokes layer, damping) or the run is linear
@@ -950,8 +981,8 @@ int hst_gpu_timestep(double deltat, double *time, double *cfl) {
     // products themselves when a closure is attached (it adds its modelled
     // stresses to them)
     const bool six = g_hook != nullptr;
-    if (!S.deck.linear) rc = nonlinear_pass(six ? 0 : 3, want, &c);
-    else if (want) rc = nonlinear_pass(2, true, &c);  // linear: the transform only for the CFL number
+    if (!S.deck.linear) rc = nonlinear_pass(six ? 0 : 3, want, &c, true);
+    else if (want) rc = nonlinear_pass(2, true, &c, true);  // linear: the transform only for the CFL number
     if (rc) return rc;
     if (!S.deck.linear && six && (rc = g_hook(g_hook_ctx, t))) return rc;
     const double dt_sub = 2.0 / ode[0] * deltat;
@@ -973,7 +1004,10 @@ int hst_gpu_timestep(double deltat, double *time, double *cfl) {
     if ((rc = stokes_apply(t))) return rc;
   }
   *time = t;
-  if (cfl) *cfl = c;
+  if (cfl) {
+    if (int rc = cfl_finish(&c)) return rc;
+    *cfl = c;
+  }
   return 0;
 }
 

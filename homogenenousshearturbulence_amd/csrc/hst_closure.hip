@@ -777,11 +777,21 @@ int stress(hst_closure *c, const double2 *const U[3], double sx, double sz, hipS
   return 0;
 }
 
+// the live state has the grid and box of the context (the sizes its kernels assume)
+bool same_grid(const StateView &v, const hst_closure *c) {
+  const Dims &F = v.D, &D = c->D;
+  return F.nx == D.nx && F.ny == D.ny && F.nz == D.nz && v.deck.alfa0 == c->deck.alfa0 &&
+         v.deck.beta0 == c->deck.beta0 && v.deck.ly == c->deck.ly && v.deck.S == c->deck.S &&
+         v.deck.ystretch == c->deck.ystretch;
+}
+
 // the timestep hook: stresses of the live state's velocity added to its products
 int hook(void *ctx, double time) {
   hst_closure *c = (hst_closure *)ctx;
   StateView v;
   if (!state_view(v)) return cfail("no live state");
+  // the state may have been freed and set up again on another grid since the attach
+  if (!same_grid(v, c)) return cfail("the live state's grid is no longer the attached context's");
   double sx, sz;
   deck_shifts(c->deck, c->g.lx, c->g.lz, time, sx, sz);
   const double2 *U[3] = {v.F[HST_V1], v.F[HST_V2], v.F[HST_V3]};
@@ -1268,11 +1278,7 @@ int hst_closure_attach(hst_closure *c, int on) {
   }
   StateView v;
   if (!state_view(v)) return cfail("attach: no live state");
-  const Dims &F = v.D, &D = c->D;
-  if (F.nx != D.nx || F.ny != D.ny || F.nz != D.nz || v.deck.alfa0 != c->deck.alfa0 ||
-      v.deck.beta0 != c->deck.beta0 || v.deck.ly != c->deck.ly || v.deck.S != c->deck.S ||
-      v.deck.ystretch != c->deck.ystretch)
-    return cfail("attach: the live state's grid is not the context's");
+  if (!same_grid(v, c)) return cfail("attach: the live state's grid is not the context's");
   if (!c->have_norm || !c->have_params) return cfail("attach: the network has no parameters or no standardisation");
   set_stress_hook(hook, c);
   g_attached = c;

@@ -319,6 +319,25 @@ def write_deck(d: Deck, path: str, comment: str = "") -> None:
             fh.write(f"&{g}  {items} /\n")
 
 
+def new_timestep(deltat: float, cfl: float, cflmax: float, dt_fixed: float) -> float:
+    """The next step length (new_timestep, src/hst.f90:163-169): cflmax/cfl
+    when cflmax and the CFL number are positive, never above the deck's fixed
+    step dt_fixed when that is set, and exactly dt_fixed without cflmax."""
+    if cflmax > 0.0 and cfl > 0.0:
+        deltat = cflmax / cfl
+    if dt_fixed > 0.0:
+        deltat = min(deltat, dt_fixed)
+    if dt_fixed > 0.0 and cflmax <= 0.0:
+        deltat = dt_fixed
+    return deltat
+
+
+def first_timestep(deltat: float) -> float:
+    """The step before the first CFL number is known: deltat = 0 in the deck
+    stands for 1 (src/hst.f90:85), which new_timestep then replaces."""
+    return 1.0 if deltat == 0.0 else deltat
+
+
 def crossed(time: float, deltat: float, dt: float) -> bool:
     """The interval test of the driver (src/hst.f90:154-158)."""
     if dt <= 0.0:

@@ -15,7 +15,7 @@ import os
 
 import numpy as np
 
-from .deck import Deck, hst_deck, hst_grid_info
+from .deck import Deck, first_timestep, hst_deck, hst_grid_info, new_timestep
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhstgpu.so")
@@ -35,6 +35,7 @@ EXPORTS = (
     "hst_gpu_timestep", "hst_gpu_outstats", "hst_gpu_set_chunk_rows", "hst_gpu_memory", "hst_gpu_profile",
     "hst_gpu_profile_count", "hst_gpu_profile_entry", "hst_gpu_pressure", "hst_gpu_spectrum",
     "hst_gpu_stokes_apply", "hst_gpu_damping_mask", "hst_gpu_outstats_extra", "hst_gpu_project_divergence",
+    "hst_gpu_bandwidth_probe",
 )
 
 
@@ -89,6 +90,7 @@ def lib() -> ctypes.CDLL:
         "hst_gpu_project_divergence": [d],
         "hst_gpu_damping_mask": [],
         "hst_gpu_outstats_extra": [dp],
+        "hst_gpu_bandwidth_probe": [d, i, dp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -341,17 +343,39 @@ class Solver:
         return rows
 
 
+def bandwidth_probe(gib: float = 4.0, reps: int = 5) -> dict:
+    """The device's practical HBM bandwidth (hst_gpu_bandwidth_probe): best
+    read-only, write-only and copy GB/s of plain streaming kernels."""
+    out = np.zeros(6)
+    _check(lib().hst_gpu_bandwidth_probe(gib, reps, _dp(out)), "hst_gpu_bandwidth_probe")
+    return {"read_GBps": round(out[0], 1), "write_GBps": round(out[1], 1), "copy_GBps": round(out[2], 1),
+            "grids": [int(out[3]), int(out[4]), int(out[5])], "GiB": gib}
+
+
 def run_deck(deck: Deck, precision: int = FP64, nsteps: int | None = None, solver: Solver | None = None) -> Solver:
-    """The driver's start-up and fixed-step time loop (src/hst.f90:63-125) on the GPU."""
+    """The driver's start-up and time loop (src/hst.f90:63-125) on the GPU:
+    fresh field, the layer profile, ghosts, the first step from the CFL
+    number of the initial field, then per step timestep() and new_timestep()
+    (src/hst.f90:163-169) until t_max or nstep (or nsteps steps when given).
+    The CFL number is computed only where it sets the step (cflmax > 0).
+    s.dts lists the step lengths taken."""
     s = solver or Solver(deck, precision)
     s.time = deck.time
     s.initial_field()
     s.stokes_apply()  # the layer profile imposed on the fresh field (src/hst.f90:76-77)
     s.fill_all_ghosts()
-    dt = deck.deltat
+    adaptive = deck.cflmax > 0.0
+    dt = first_timestep(deck.deltat)
+    if adaptive:
+        s.cfl = s.compute_cfl()
+    dt = new_timestep(dt, s.cfl, deck.cflmax, deck.dt_fixed)
     istep = 0
+    s.dts = []
     while (s.time < deck.t_max - 0.5 * dt and istep < deck.nstep) if nsteps is None else istep < nsteps:
-        s.timestep(dt)
+        s.timestep(dt, want_cfl=adaptive)
+        s.dts.append(dt)
         istep += 1
+        dt = new_timestep(dt, s.cfl, deck.cflmax, deck.dt_fixed)
     s.nsteps = istep
+    s.deltat = dt
     return s

@@ -227,6 +227,16 @@ int hst_gpu_profile(int enable);
 int hst_gpu_profile_count(void);
 int hst_gpu_profile_entry(int i, char *name, int len, int64_t *launches, double *ms, double *bytes);
 
+/* The device's practical HBM bandwidth, measured (no counterpart in the
+ * reference: the yardstick of the per-kernel GB/s).  Streaming kernels with no
+ * arithmetic over buffers of `gib` GiB (0.25..64; far above the 256 MB last-level
+ * cache for gib >= 4), four independent 16-byte loads in flight per lane,
+ * non-temporal stores, several grid sizes, best of `reps` timed launches each:
+ * out[0..2] = read-only, write-only and copy GB/s (copy counts bytes read plus
+ * bytes written), out[3..5] the grid (blocks of 256) of each best.  Needs no
+ * hst_gpu_init; allocates and frees its own buffers. */
+int hst_gpu_bandwidth_probe(double gib, int reps, double out[6]);
+
 #ifdef __cplusplus
 }
 #endif

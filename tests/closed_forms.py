@@ -243,3 +243,31 @@ def stokes_layer(make_solver, deck):
     V[2, 0, nz] = 0.0
     other = max(other, float(np.abs(V).max()))
     return err / float(np.abs(wex).max()), other
+
+
+def cfl_single_mode(deck, y, au=0.7, aw=0.4, peak_row=None, ghost_amp=50.0):
+    """A field whose CFL number is known in closed form (compute_cfl,
+    src/hst_transforms.f90:163-179): u, v, w all in the Fourier mode
+    (ix, iz) = (1, 0) with real amplitudes, so that each is 2 a cos(alfa0 x)
+    and all three peak together at x = 0, a physical point.  u and w have the
+    same amplitude on every row; v has amplitude 1 on row peak_row only (an
+    interior row, by default the row of smallest dyl), a much larger one on the
+    ghost rows (which the CFL number must not see) and 0.25 elsewhere.  The
+    CFL number is then 2 au/dx + 2 aw/dz + max over interior rows of
+    2 av(iy)/dyl(iy), with dx = lx/(2 nxd), dz = lz/nzd.
+    Returns (V, expected)."""
+    ny = deck.ny
+    nxd, nzd = deck.nxd, deck.nzd
+    dyl = np.array([0.5 * (y[iy + 3] - y[iy + 1]) for iy in range(ny)])
+    if peak_row is None:
+        peak_row = int(np.argmin(dyl))
+    av = np.full(ny + 4, 0.25)
+    av[peak_row + 2] = 1.0
+    av[[0, 1, ny + 2, ny + 3]] = ghost_amp
+    V = np.zeros((3, deck.nx + 1, 2 * deck.nz + 1, ny + 4), dtype=np.complex128)
+    V[0, 1, deck.nz, :] = au
+    V[1, 1, deck.nz, :] = av
+    V[2, 1, deck.nz, :] = aw
+    dx, dz = deck.lx / (2 * nxd), deck.lz / nzd
+    expected = 2 * au / dx + 2 * aw / dz + float(np.max(2 * av[2:ny + 2] / dyl))
+    return V, expected

@@ -101,3 +101,39 @@ def random_field(deck, seed_re=3, seed_im=4, uniform=None):
 
 def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def physical_velocity(deck, V, nxd, nzd):
+    """u, v, w at the 2nxd x nzd points of every row (-2..ny+1) by numpy's own
+    FFTs, independent of the oracle's and the library's transforms: the z
+    inverse (+i, unnormalised) with mode iz at z index iz mod nzd, then the x
+    complex-to-real transform of length 2nxd (unnormalised), which, as every
+    c2r does, takes only the real part of the kx = 0 bin.  (3, ny+4, nzd, 2nxd)."""
+    nx, nz, ny = deck.nx, deck.nz, deck.ny
+    out = np.empty((3, ny + 4, nzd, 2 * nxd))
+    for c in range(3):
+        Z = np.zeros((nx + 1, nzd, ny + 4), dtype=np.complex128)
+        for iz in range(-nz, nz + 1):
+            Z[:, iz % nzd, :] = V[c][:, iz + nz, :]
+        Z = np.fft.ifft(Z, axis=1) * nzd
+        X = np.zeros((nxd + 1, nzd, ny + 4), dtype=np.complex128)
+        X[:nx + 1] = Z
+        phys = np.fft.irfft(X, n=2 * nxd, axis=0) * (2 * nxd)
+        out[c] = phys.transpose(2, 1, 0)
+    return out
+
+
+def cfl_bruteforce(deck, V, y, nxd, nzd):
+    """max over the interior rows 0..ny-1 and every physical point of
+    |u|/dx + |v|/dyl + |w|/dz: dx = lx/(2nxd) and dz = lz/nzd the spacings of
+    the physical points, dyl(iy) = (y(iy+1) - y(iy-1))/2 from the grid's own
+    rows (ghosts included: the periodic extension)."""
+    ny = deck.ny
+    phys = physical_velocity(deck, V, nxd, nzd)
+    dx, dz = deck.lx / (2 * nxd), deck.lz / nzd
+    best = 0.0
+    for iy in range(ny):
+        dyl = 0.5 * (y[iy + 3] - y[iy + 1])
+        u, v, w = phys[0, iy + 2], phys[1, iy + 2], phys[2, iy + 2]
+        best = max(best, float((np.abs(u) / dx + np.abs(v) / dyl + np.abs(w) / dz).max()))
+    return best

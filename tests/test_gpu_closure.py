@@ -279,3 +279,34 @@ def test_steps_across_attach_and_detach(filtered):
             s.close()
     print("attach/detach sequence, with and without reads between", rel(res[0], res[1]))
     assert rel(res[0], res[1]) < 1e-12
+
+
+def test_free_and_init_detach_the_closure(O, filtered):
+    """A closure attached to a solver state is detached when that state is
+    freed: a state set up afterwards (here on another grid, where the
+    closure's sizes would not fit its buffers) steps as the plain method,
+    equal to the oracle's plain steps."""
+    from conftest import deck_path
+
+    f = filtered
+    cd = f["cd"]
+    nm = norm_of(f["X"].astype(np.float64), f["Y"].astype(np.float64))
+    c = closure.Closure(cd, 0)
+    try:
+        c.set_params(random_params(6, scale=0.2))
+        c.set_norm(nm)
+        s = hst.Solver(cd, hst.FP64)
+        c.attach(True)
+        s.close()
+        d2 = read_input(deck_path("small"))
+        g = hst.run_deck(d2, hst.FP64, nsteps=3)
+        try:
+            Vg = g.V
+        finally:
+            g.close()
+    finally:
+        c.close()
+    o = O.run_deck(d2, nsteps=3)
+    e = rel(Vg[..., 2:-2], o.V[..., 2:-2])
+    print("plain steps after the freed state vs oracle", e)
+    assert e < 1e-10

@@ -444,3 +444,97 @@ def test_box32_fp64_100_steps_match_oracle(O):
         assert worst_series <= 1e-10
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("precision,tol", [(hst.FP64, 1e-12), (hst.MIXED, 2e-5)])
+def test_products_z1536(O, precision, tol):
+    """Products vs the oracle at the z length of the nx = nz = 512, ny = 1025
+    workload (nzd = 1536) on a thin deck (nx = 6, ny = 12)."""
+    d = dataclasses.replace(read_input(deck_path("small")), nx=6, nz=512, ny=12, kpeak=3.0)
+    assert d.nzd == 1536
+    o = seeded(O, d, 0.11)
+    o.nonlinear(mode=0)
+    P0 = o.products()
+    with GPU(d, precision) as g:
+        g.set_V(o.V)
+        g.nonlinear()
+        for p in range(6):
+            assert rel(g.product(p + 1), P0[p]) <= tol, p
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_one_step_z1536(O, exact):
+    """One FP64 step at nzd = 1536 (thin x and y) against the oracle, with and
+    without the exact shift."""
+    d = dataclasses.replace(read_input(deck_path("small")), nx=6, nz=512, ny=12, kpeak=3.0, alfa0=1.0,
+                            beta0=1.0, ly=2 * np.pi, re=1000.0, amplitude=0.002, deltat=0.001, exact_shift=exact)
+    o = O.run_deck(d, nsteps=1)
+    g = hst.run_deck(d, hst.FP64, nsteps=1)
+    try:
+        assert g.time == o.time
+        assert rel(g.V, o.V) <= 1e-11
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("ystretch", [0.0, 2.0])
+def test_cfl_closed_form(ystretch):
+    """compute_cfl on the GPU against the single-mode closed form (the CFL
+    number of u, v, w in one Fourier mode, ghost rows excluded), uniform and
+    stretched grid."""
+    d = dataclasses.replace(read_input(deck_path("small")), ystretch=ystretch)
+    with GPU(d) as g:
+        V, want = closed_forms.cfl_single_mode(d, g.y)
+        g.set_V(V)
+        got = g.compute_cfl()
+    assert abs(got - want) <= 1e-12 * want, (got, want)
+
+
+@pytest.mark.parametrize("name", ["small", "small_s2x"])
+def test_adaptive_steps_match_oracle(O, name):
+    """The reference's default time control (deltat = 0, cflmax = 1.0,
+    src/hst_input.f90:32; new_timestep, src/hst.f90:163-169): 50 steps whose
+    lengths come from the CFL number of each step's third substep, GPU vs
+    oracle.  The two CFL numbers come from different FFTs, so the step lengths
+    and the clock agree to rounding (1e-13), not bit for bit; fields 1e-10.
+    small_s2x has the exact shift, whose factor tables are rebuilt for every
+    new lambda."""
+    d = dataclasses.replace(read_input(deck_path(name)), deltat=0.0, cflmax=1.0)
+    o = O.run_deck(d, nsteps=50)
+    g = hst.run_deck(d, hst.FP64, nsteps=50)
+    try:
+        dts = np.array(g.dts)
+        assert len(g.dts) == 50 and len(set(g.dts)) > 40
+        assert np.abs(dts - np.array(o.dts)).max() <= 1e-13 * dts.max()
+        assert abs(g.time - o.time) <= 1e-13 * o.time
+        print(name, "t", g.time, "dt", dts.min(), dts.max(), "cfl", g.cfl, o.cfl)
+        assert rel(g.V, o.V) <= 1e-10
+        assert abs(g.cfl - o.cfl) <= 1e-12 * o.cfl
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("precision,tol", [(hst.FP64, 1e-10), (hst.MIXED, 1e-5)])
+def test_exact_shift_without_factor_tables(O, precision, tol):
+    """The exact-shift time step without factor tables (HST_NO_TABLES=1 at
+    init; the path taken when they do not fit in device memory): the rhs shift
+    apart from the solves, the solves through the full workspace; 10 steps of
+    small_s2x against the oracle and against the tabled path."""
+    d = read_input(deck_path("small_s2x"))
+    o = O.run_deck(d, nsteps=10)
+    os.environ["HST_NO_TABLES"] = "1"
+    try:
+        g = hst.run_deck(d, precision, nsteps=10)
+    finally:
+        del os.environ["HST_NO_TABLES"]
+    try:
+        assert g.time == o.time
+        Vn = g.V
+        assert refops.rel_l2(Vn, o.V) <= tol
+    finally:
+        g.close()
+    g = hst.run_deck(d, precision, nsteps=10)
+    try:
+        assert refops.rel_l2(g.V, Vn) <= tol
+    finally:
+        g.close()
