@@ -888,6 +888,119 @@ int hst_gpu_product_to_field(int p, int field) {
   return 0;
 }
 
+// ------------------------------------------------------------------ per-field transforms of host buffers
+
+// A device buffer for one call of the per-field entry points (they run
+// outside the time step, on the reference's buffer shapes).
+struct TmpBuf {
+  void *p = nullptr;
+  ~TmpBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// exp(-2 pi i k / den) for k = 0 .. n-1, FP64, into a new device buffer
+static int twiddles64(TmpBuf &b, int n, double den) {
+  std::vector<double2> h((size_t)n);
+  for (int k = 0; k < n; k++) {
+    const double a = -TWOPI * (double)k / den;
+    h[(size_t)k] = make_double2(std::cos(a), std::sin(a));
+  }
+  HIPCHK(hipMalloc(&b.p, (size_t)n * sizeof(double2)));
+  HIPCHK(hipMemcpy(b.p, h.data(), (size_t)n * sizeof(double2), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// nlines lines of n complex (host), transformed on the device in place
+static int lines_on_device(double *lines, int64_t nlines, size_t line_bytes,
+                           const std::function<int(double2 *)> &launch) {
+  if (!lines || nlines < 0) return fail("lines is NULL or nlines < 0");
+  if (nlines == 0) return 0;
+  const size_t bytes = (size_t)nlines * line_bytes;
+  TmpBuf d;
+  HIPCHK(hipMalloc(&d.p, bytes));
+  HIPCHK(hipMemcpyAsync(d.p, lines, bytes, hipMemcpyHostToDevice, S.stream));
+  const int rc = launch((double2 *)d.p);
+  if (rc) return hip_fail("per-field transform", (hipError_t)rc);
+  HIPCHK(hipMemcpyAsync(lines, d.p, bytes, hipMemcpyDeviceToHost, S.stream));
+  HIPCHK(hipStreamSynchronize(S.stream));
+  return 0;
+}
+
+int hst_gpu_fft_z(double *lines, int64_t nlines, int sign) {
+  if (int rc = check_live_keep()) return rc;
+  if (sign != 1 && sign != -1) return fail("fft_z: sign must be +1 (IFT) or -1 (FFT)");
+  const int n = S.D.nzd;
+  TmpBuf tw;
+  if (int rc = twiddles64(tw, n, n)) return rc;
+  return lines_on_device(lines, nlines, (size_t)n * sizeof(double2),
+                         [&](double2 *d) { return launch_cfft_lines(n, sign, d, nlines, tw.p, S.stream); });
+}
+
+int hst_gpu_fft_x(double *lines, int64_t nlines, int dir) {
+  if (int rc = check_live_keep()) return rc;
+  if (dir != 1 && dir != -1) return fail("fft_x: dir must be +1 (RFT, c2r) or -1 (HFT, r2c)");
+  const int m = S.D.nxd;
+  TmpBuf tw, twh;
+  if (int rc = twiddles64(tw, m, m)) return rc;
+  if (int rc = twiddles64(twh, m + 1, 2.0 * m)) return rc;
+  return lines_on_device(lines, nlines, (size_t)(m + 1) * sizeof(double2), [&](double2 *d) {
+    return launch_rfft_lines(m, dir, d, nlines, tw.p, twh.p, S.stream);
+  });
+}
+
+int hst_gpu_products(int g, int64_t nlines, const double *vel, double *prod) {
+  if (int rc = check_live_keep()) return rc;
+  if (g != 1 && g != 2) return fail("products: group must be 1 or 2");
+  if (!vel || !prod || nlines < 0) return fail("products: NULL buffer or nlines < 0");
+  if (nlines == 0) return 0;
+  const int lw = 2 * (S.D.nxd + 1);
+  const int64_t plane = nlines * lw;
+  const size_t bytes = (size_t)plane * 3 * sizeof(double);
+  TmpBuf dv, dp;
+  HIPCHK(hipMalloc(&dv.p, bytes));
+  HIPCHK(hipMalloc(&dp.p, bytes));
+  HIPCHK(hipMemcpyAsync(dv.p, vel, bytes, hipMemcpyHostToDevice, S.stream));
+  HIPCHK(hipMemcpyAsync(dp.p, prod, bytes, hipMemcpyHostToDevice, S.stream));
+  const int rc = launch_products((const double *)dv.p, (double *)dp.p, plane, lw, 2 * S.D.nxd, g, S.D.factor,
+                                 S.stream);
+  if (rc) return hip_fail("products", (hipError_t)rc);
+  HIPCHK(hipMemcpyAsync(prod, dp.p, bytes, hipMemcpyDeviceToHost, S.stream));
+  HIPCHK(hipStreamSynchronize(S.stream));
+  return 0;
+}
+
+int hst_gpu_set_products(int g, const double *vvdz) {
+  if (int rc = check_live_keep()) return rc;
+  if (g != 1 && g != 2) return fail("set_products: group must be 1 or 2");
+  if (!vvdz) return fail("set_products: vvdz is NULL");
+  const Dims &D = S.D;
+  const size_t nf = (size_t)D.NYg * D.ncol;
+  const double2 *src = (const double2 *)vvdz;
+  for (int m = 0; m < 3; m++) {
+    // VVdz(jz, ix + 1, row, m): jz = izd(iz) + 1, izd(iz) = iz mod nzd
+    std::vector<double2> h(nf);
+    for (int row = 0; row < D.NYg; row++)
+      for (int ix = 0; ix < D.NXm; ix++)
+        for (int iz = -D.nz; iz <= D.nz; iz++) {
+          const int jz = iz >= 0 ? iz : D.nzd + iz;
+          h[(size_t)row * D.ncol + (size_t)ix * D.NZm + (iz + D.nz)] =
+              src[(((size_t)m * D.NYg + row) * D.NXm + ix) * D.nzd + jz];
+        }
+    void *dst = (void *)P_of(3 * (g - 1) + m);
+    if (S.mixed) {
+      std::vector<float2> f(nf);
+      for (size_t i = 0; i < nf; i++) f[i] = make_float2((float)h[i].x, (float)h[i].y);
+      HIPCHK(hipMemcpyAsync(dst, f.data(), nf * sizeof(float2), hipMemcpyHostToDevice, S.stream));
+      HIPCHK(hipStreamSynchronize(S.stream));
+    } else {
+      HIPCHK(hipMemcpyAsync(dst, h.data(), nf * sizeof(double2), hipMemcpyHostToDevice, S.stream));
+      HIPCHK(hipStreamSynchronize(S.stream));
+    }
+  }
+  return 0;
+}
+
 int hst_gpu_buildrhs_prepare(const double ode[3], double deltat, double time) {
   if (int rc = check_live()) return rc;
   // the substep's damping_update precedes buildrhs_prepare (src/hst_equations.f90:64)

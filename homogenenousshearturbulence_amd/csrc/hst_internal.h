@@ -119,6 +119,12 @@ int launch_d0solve(const Dims &D, int rhs_mode, int out_mode, const double2 *src
                    double2 *XR, const double *f0rows, const D0Const &fc, const double *der, const double *dyl,
                    double sx, double sz, double *part, int *n_part, hipStream_t s, const int *inlayer = nullptr,
                    int skip00 = 0);
+// batched FP64 line transforms of the per-field entry points (hst_lines.hip)
+int launch_cfft_lines(int n, int sign, double2 *data, int64_t nlines, const void *tw, hipStream_t s);
+int launch_rfft_lines(int m, int dir, double2 *data, int64_t nlines, const void *tw, const void *twh,
+                      hipStream_t s);
+int launch_products(const double *vel, double *prod, int64_t plane, int lw, int used, int g, double factor,
+                    hipStream_t s);
 
 // The exact shift of q[0..nq) (rhs1, rhs2, oldrhs1, oldrhs2): D0 solve at the
 // old shifts, node phase, ghosts at the new shifts, D0 re-weight.

@@ -1,124 +1,162 @@
-! Direct numerical simulation of homogeneous shear turbulence: the driver of
-! the reference (src/hst.f90) on one MI355X.  Usage:  hst [hst.in]
-! Set-up, then per step: timestep(), statistics, snapshots (velocity,
-! pressure and the averaged column-energy spectrum), restart file, new
-! time step from the CFL number.  The run ends at t_max, after nstep steps or
-! after wall_max seconds, whichever comes first.
+! hst on one MI355X: the run driver.   Usage:  hst [deck]   (default hst.in)
+!
+! The state stays on the GPU for the whole run (allocate_fields with
+! resident = .true.); the host sees the velocity only when it writes a file.
+! Output follows the reference's schedule (src/hst.f90:93-125): a line of
+! statistics at the start and whenever the step crosses a multiple of
+! dt_stat, a snapshot (velocity, pressure, column-energy spectrum) at each
+! multiple of dt_field, the restart file Dati.cart.out at each multiple of
+! dt_save and at the end; the step length follows the CFL number (cflmax)
+! or stays at the deck's deltat.  The run stops at t_max, after nstep steps
+! or after wall_max seconds in the loop.
 program hst
 
   use, intrinsic :: iso_c_binding
   use hst_params
-  use hst_input
-  use hst_setup
-  use hst_fft
-  use hst_derivatives
-  use hst_linsolve
-  use hst_transforms
-  use hst_equations
+  use hst_gpu_api
+  use hst_input, only: read_input, print_input
+  use hst_setup, only: select_device, allocate_fields, free_fields
+  use hst_fft, only: init_fft, free_fft, device_sync
+  use hst_derivatives, only: setup_derivatives, fill_ghosts
+  use hst_linsolve, only: init_linsolve, free_linsolve
+  use hst_equations, only: timestep
   use hst_initial, only: generate_initial_field
-  use hst_io
-  use hst_stats
-  use hst_pressure
-  use hst_timer
-  use hst_stokes, only: stokes_apply
+  use hst_io, only: restart_read, restart_write, make_output_dirs
+  use hst_stats, only: outstats, open_runtimedata, close_runtimedata, spectrum_write
+  use hst_pressure, only: write_pressure
+  use hst_timer, only: timer_start, tic, toc, timer_report, T_OTHER
+  use hst_stokes, only: stokes_setup, stokes_apply
+  use hst_damping, only: damping_setup, damping_active
 
   implicit none
 
-  character(len=256) :: deck
-  character(len=40) :: fname
-  integer(8) :: c0, c1, rate
-  real(C_DOUBLE) :: elapsed
-  logical :: wall_reached = .false.
+  real(C_DOUBLE) :: loop_seconds = 0.0d0
+  logical :: out_of_time = .false.
 
-  call select_device()
-  deck = 'hst.in'
-  if (command_argument_count() >= 1) call get_command_argument(1, deck)
-  call read_input(trim(deck))
-  if (has_terminal) call print_input()
-  call allocate_fields()
-  call init_fft()
-  call init_linsolve()
-  call setup_derivatives()
-  call make_output_dirs()
-  if (restart_from_file) then
-    call restart_read(trim(restart_file))
-  else
-    if (has_terminal) print *, '   restart_from_file = .false.: generating initial field'
-    call generate_initial_field()
-  end if
-  ! the Stokes layer's profile imposed on the loaded field (stokes_setup, stokes_apply)
-  call stokes_apply()
-  call fill_ghosts(1)
-  call fill_ghosts(2)
-  call fill_ghosts(3)
-  call open_runtimedata()
-
-  ! first time step from the CFL of the initial field
-  if (deltat == 0.0d0) deltat = 1.0d0
-  call compute_cfl()
-  call new_timestep()
-  ifield = floor(time/dt_field)
-  if (has_terminal) write (*, '(A)') '        time       deltat       cfl         energy           diss' // &
-    '             uv             vw      (y-averaged; diss = nu <grad u : grad u>)'
-  call outstats()
-
-  elapsed = 0.0d0
-  do while (time < t_max - 0.5d0*deltat .and. istep < nstep .and. .not. wall_reached)
-    call system_clock(c0, rate)
-    istep = istep + 1
-    call tic()
-    call timestep()
-    call toc(T_STEP)
-    if (crossed(dt_stat)) call outstats()
-    if (crossed(dt_field)) then
-      ifield = ifield + 1
-      write (fname, '(A,I0,A)') 'fields/field', ifield, '.fld'
-      if (has_terminal) print '(A,F12.5)', '   writing '//trim(fname)//' at time', time
-      call restart_write(trim(fname))
-      write (fname, '(A,I0,A)') 'p_fields/pField', ifield, '.fld'
-      call write_pressure(trim(fname))
-      write (fname, '(A,I0,A)') 'statistics/spectra/exz', ifield, '.dat'
-      call spectrum_write(trim(fname))
-    end if
-    if (crossed(dt_save)) then
-      if (has_terminal) print '(A,F12.5)', '   writing Dati.cart.out at time', time
-      call restart_write('Dati.cart.out')
-    end if
-    call new_timestep()
-    call toc(T_OTHER)
-    call device_sync()
-    call system_clock(c1)
-    elapsed = elapsed + real(c1 - c0, C_DOUBLE)/real(rate, C_DOUBLE)
-    if (wall_max > 0.0d0) wall_reached = elapsed > wall_max
-    if (has_terminal .and. (mod(istep, 50_C_SIZE_T) == 0 .or. istep <= 5)) &
-      write (*, '(A,I0,A,F9.5,A,F12.2,A)') '   step ', istep, ': ', real(c1 - c0, C_DOUBLE)/real(rate, C_DOUBLE), &
-      ' s/step, ', elapsed, ' s elapsed'
+  call start()
+  do while (keep_going())
+    call one_step()
   end do
-
-  if (has_terminal) print '(A,F12.5,A,I0,A,F10.2,A,F9.5,A)', '   end of run at time', time, ' after ', istep, &
-    ' steps; ', elapsed, ' s, ', elapsed/max(istep, 1_C_SIZE_T), ' s/step; writing Dati.cart.out'
-  if (has_terminal .and. wall_reached) print '(A,F10.0,A)', '   (wall-clock limit wall_max =', wall_max, ' s reached)'
-  call timer_report(istep)
-  call restart_write('Dati.cart.out')
-  call close_runtimedata()
-  call free_linsolve()
-  call free_fft()
-  call free_fields()
+  call finish()
 
 contains
 
-  logical function crossed(dt)
-    real(C_DOUBLE), intent(in) :: dt
-    crossed = .false.
-    if (dt > 0.0d0) crossed = floor((time + 0.5d0*deltat)/dt) > floor((time - 0.5d0*deltat)/dt)
-  end function crossed
+  subroutine start()
+    character(len=256) :: deck
+    integer(C_INT) :: c
+    call select_device()
+    deck = 'hst.in'
+    if (command_argument_count() > 0) call get_command_argument(1, deck)
+    call read_input(trim(deck))
+    if (has_terminal) call print_input()
+    call allocate_fields(resident=.true.)
+    call init_fft()
+    call init_linsolve()
+    call setup_derivatives()
+    call damping_setup()
+    call make_output_dirs()
+    if (restart_from_file) then
+      call restart_read(trim(restart_file))
+    else
+      call generate_initial_field()
+    end if
+    call stokes_setup()
+    call stokes_apply()
+    do c = 1, 3
+      call fill_ghosts(c)
+    end do
+    call open_runtimedata()
+    ! the first step from the CFL number of the start field
+    if (deltat == 0.0d0) deltat = 1.0d0
+    call gpu_check(hst_gpu_compute_cfl(cfl), 'compute_cfl')
+    deltat = step_length(deltat, cfl)
+    if (has_terminal .and. damping_active() .and. .not. damp_hard .and. damp_rate*deltat > 1.0d0) &
+      print '(A,ES10.3,A)', '   note: the explicit damping takes damp_rate*deltat = ', damp_rate*deltat, ' > 1'
+    ifield = floor(time/dt_field)
+    if (has_terminal) print '(/,A,/,A)', '        time       deltat       cfl         energy           diss' // &
+      '             uv             vw', '   (y-averaged; diss = nu <grad u : grad u>)'
+    call outstats()
+    call timer_start()
+  end subroutine start
 
-  ! next time step: from cflmax when the deck gives deltat = 0, else the fixed
-  ! value (capped by cflmax if that is set)
-  subroutine new_timestep()
-    if (cflmax > 0.0d0 .and. cfl > 0.0d0) deltat = cflmax/cfl
-    if (dt_fixed > 0.0d0) deltat = min(deltat, dt_fixed)
-    if (dt_fixed > 0.0d0 .and. cflmax <= 0.0d0) deltat = dt_fixed
-  end subroutine new_timestep
+  logical function keep_going()
+    keep_going = time < t_max - 0.5d0*deltat .and. istep < nstep .and. .not. out_of_time
+  end function keep_going
+
+  subroutine one_step()
+    integer(8) :: c0, c1, rate
+    call system_clock(c0, rate)
+    istep = istep + 1
+    call timestep()
+    call tic()
+    call write_what_is_due()
+    deltat = step_length(deltat, cfl)
+    call toc(T_OTHER)
+    call device_sync()
+    call system_clock(c1)
+    loop_seconds = loop_seconds + real(c1 - c0, C_DOUBLE)/real(rate, C_DOUBLE)
+    if (wall_max > 0.0d0) out_of_time = loop_seconds > wall_max
+    if (has_terminal .and. (istep <= 5 .or. mod(istep, 50_C_SIZE_T) == 0)) &
+      print '(A,I0,A,F9.5,A,F11.2,A)', '   step ', istep, ' took ', real(c1 - c0, C_DOUBLE)/real(rate, C_DOUBLE), &
+      ' s (', loop_seconds, ' s in the loop)'
+  end subroutine one_step
+
+  ! statistics, snapshot and restart file whose interval ends at this step
+  subroutine write_what_is_due()
+    character(len=48) :: name
+    if (due(dt_stat)) call outstats()
+    if (due(dt_field)) then
+      ifield = ifield + 1
+      write (name, '(A,I0,A)') 'fields/field', ifield, '.fld'
+      if (has_terminal) print '(A,F12.5)', '   snapshot '//trim(name)//' at t =', time
+      call restart_write(trim(name))
+      write (name, '(A,I0,A)') 'p_fields/pField', ifield, '.fld'
+      call write_pressure(trim(name))
+      write (name, '(A,I0,A)') 'statistics/spectra/exz', ifield, '.dat'
+      call spectrum_write(trim(name))
+    end if
+    if (due(dt_save)) then
+      if (has_terminal) print '(A,F12.5)', '   restart file Dati.cart.out at t =', time
+      call restart_write('Dati.cart.out')
+    end if
+  end subroutine write_what_is_due
+
+  ! .true. when a multiple of period lies within half a step of the clock
+  logical function due(period)
+    real(C_DOUBLE), intent(in) :: period
+    due = .false.
+    if (period <= 0.0d0) return
+    due = floor((time + 0.5d0*deltat)/period) > floor((time - 0.5d0*deltat)/period)
+  end function due
+
+  ! The next step length (src/hst.f90:163-169): cflmax/cfl when cflmax is
+  ! set, at most the deck's deltat when that is set too; the deck's deltat
+  ! alone without cflmax; otherwise the step stays.
+  pure real(C_DOUBLE) function step_length(dt, c) result(d)
+    real(C_DOUBLE), intent(in) :: dt, c
+    d = dt
+    if (cflmax > 0.0d0 .and. c > 0.0d0) d = cflmax/c
+    if (dt_fixed > 0.0d0) then
+      if (cflmax > 0.0d0) then
+        d = min(d, dt_fixed)
+      else
+        d = dt_fixed
+      end if
+    end if
+  end function step_length
+
+  subroutine finish()
+    if (has_terminal) then
+      print '(A,F12.5,A,I0,A,F10.2,A,F9.5,A)', '   run ends at t =', time, ' after ', istep, ' steps, ', &
+        loop_seconds, ' s in the loop (', loop_seconds/max(istep, 1_C_SIZE_T), ' s/step)'
+      if (out_of_time) print '(A,F10.0,A)', '   the loop stopped at wall_max =', wall_max, ' s'
+    end if
+    call timer_report(istep)
+    call restart_write('Dati.cart.out')
+    call close_runtimedata()
+    call free_linsolve()
+    call free_fft()
+    call free_fields()
+  end subroutine finish
 
 end program hst

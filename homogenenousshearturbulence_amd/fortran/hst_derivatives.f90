@@ -1,29 +1,33 @@
-! Compact finite differences in y and the shear-periodic ghost rows
-! (the reference's module hst_derivatives).  The weights der(iy, n, j) are
-! computed by the library at set-up (same construction, same bits); the
-! clock helpers S2(t), its integral and the wrap shifts are evaluated here.
+! Compact finite differences in y and the shear-periodic ghost rows (the
+! reference's module hst_derivatives, src/hst_derivatives.f90).  The weights
+! der(iy, n, j) come from the library at set-up (the same construction and
+! bits); the clock helpers S2(t), its integral and the wrap shifts are the
+! closed forms the reference evaluates on the host.
 module hst_derivatives
 
   use, intrinsic :: iso_c_binding
   use hst_params
   use hst_gpu_api
+  use hst_setup, only: to_device, from_device
 
   implicit none
   private
   public :: setup_derivatives, fill_ghosts, fill_ghosts_field, shear_shifts
   public :: s2_of, s2_integral, gamma_y_of
 
+  real(C_DOUBLE), parameter :: TWOPI = 6.283185307179586d0
+
 contains
 
   ! der, y, dyl were filled by allocate_fields from the library's tables.
   subroutine setup_derivatives()
-    if (ny < 8) error stop 'ny must be at least 8'
+    if (ny < 8) error stop 'setup_derivatives: ny must be at least 8'
     if (.not. allocated(der)) error stop 'setup_derivatives: call allocate_fields first'
   end subroutine setup_derivatives
 
+  ! S2(t) = s2_mean + s2_amplitude sin(2 pi (t - s2_start)/s2_period) from s2_start on
   real(C_DOUBLE) function s2_of(t)
     real(C_DOUBLE), intent(in) :: t
-    real(C_DOUBLE), parameter :: TWOPI = 6.283185307179586d0
     s2_of = 0.0d0
     if ((s2_amplitude == 0.0d0 .and. s2_mean == 0.0d0) .or. t < s2_start) return
     if (s2_period > 0.0d0) then
@@ -33,9 +37,9 @@ contains
     end if
   end function s2_of
 
+  ! the integral of S2 from t1 to t2 (the part after s2_start)
   real(C_DOUBLE) function s2_integral(t1, t2)
     real(C_DOUBLE), intent(in) :: t1, t2
-    real(C_DOUBLE), parameter :: TWOPI = 6.283185307179586d0
     real(C_DOUBLE) :: a, b, omega
     s2_integral = 0.0d0
     if ((s2_amplitude == 0.0d0 .and. s2_mean == 0.0d0) .or. t2 <= s2_start) return
@@ -53,6 +57,7 @@ contains
     gamma_y_of = s2_integral(s2_start, t)
   end function gamma_y_of
 
+  ! displacements of the upper image at time t
   subroutine shear_shifts(t, shift_x, shift_z)
     real(C_DOUBLE), intent(in) :: t
     real(C_DOUBLE), intent(out) :: shift_x, shift_z
@@ -64,17 +69,28 @@ contains
   subroutine fill_ghosts(c, c2)
     integer(C_INT), intent(in) :: c
     integer(C_INT), intent(in), optional :: c2
-    real(C_DOUBLE) :: shift_x, shift_z
-    call shear_shifts(time, shift_x, shift_z)
-    call gpu_check(hst_gpu_fill_ghosts(c, shift_x, shift_z), 'fill_ghosts')
-    if (present(c2)) call gpu_check(hst_gpu_fill_ghosts(c2, shift_x, shift_z), 'fill_ghosts')
+    real(C_DOUBLE) :: sx, sz
+    call shear_shifts(time, sx, sz)
+    call ghosts_of_component(c, sx, sz)
+    if (present(c2)) call ghosts_of_component(c2, sx, sz)
   end subroutine fill_ghosts
 
-  ! Ghost rows of any device field (id) for the displacements shift_x, shift_z.
+  subroutine ghosts_of_component(c, sx, sz)
+    integer(C_INT), intent(in) :: c
+    real(C_DOUBLE), intent(in) :: sx, sz
+    if (.not. state_resident) call to_device(HST_V1 + c - 1, V(:, :, :, c))
+    call gpu_check(hst_gpu_fill_ghosts(HST_V1 + c - 1, sx, sz), 'fill_ghosts')
+    if (.not. state_resident) call from_device(HST_V1 + c - 1, V(:, :, :, c))
+  end subroutine ghosts_of_component
+
+  ! Ghost rows of any array of the field shape for the displacements
+  ! shift_x, shift_z (the work runs in the library's spare field).
   subroutine fill_ghosts_field(field, shift_x, shift_z)
-    integer(C_INT), intent(in) :: field
+    complex(C_DOUBLE_COMPLEX), intent(inout) :: field(-2:ny + 1, -nz:nz, 0:nx)
     real(C_DOUBLE), intent(in) :: shift_x, shift_z
-    call gpu_check(hst_gpu_fill_ghosts(field, shift_x, shift_z), 'fill_ghosts_field')
+    call to_device(HST_SCRATCH, field)
+    call gpu_check(hst_gpu_fill_ghosts(HST_SCRATCH, shift_x, shift_z), 'fill_ghosts_field')
+    call from_device(HST_SCRATCH, field)
   end subroutine fill_ghosts_field
 
 end module hst_derivatives

@@ -173,6 +173,48 @@ module hst_gpu_api
       integer(c_int64_t), intent(out) :: bytes
       integer(c_int), intent(out) :: chunk
     end function
+    ! per-field transforms of the reference's buffers (host arrays)
+    integer(c_int) function hst_gpu_fft_z(lines, nlines, sign) bind(c, name='hst_gpu_fft_z')
+      import :: c_int, c_int64_t, c_double_complex
+      complex(c_double_complex), intent(inout) :: lines(*)
+      integer(c_int64_t), value :: nlines
+      integer(c_int), value :: sign
+    end function
+    integer(c_int) function hst_gpu_fft_x(lines, nlines, dir) bind(c, name='hst_gpu_fft_x')
+      import :: c_int, c_int64_t, c_double_complex
+      complex(c_double_complex), intent(inout) :: lines(*)
+      integer(c_int64_t), value :: nlines
+      integer(c_int), value :: dir
+    end function
+    integer(c_int) function hst_gpu_products(g, nlines, vel, prod) bind(c, name='hst_gpu_products')
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: g
+      integer(c_int64_t), value :: nlines
+      real(c_double), intent(in) :: vel(*)
+      real(c_double), intent(inout) :: prod(*)
+    end function
+    integer(c_int) function hst_gpu_set_products(g, vvdz) bind(c, name='hst_gpu_set_products')
+      import :: c_int, c_double_complex
+      integer(c_int), value :: g
+      complex(c_double_complex), intent(in) :: vvdz(*)
+    end function
+    ! per-kernel timing (HIP events on the library's stream)
+    integer(c_int) function hst_gpu_profile(enable) bind(c, name='hst_gpu_profile')
+      import :: c_int
+      integer(c_int), value :: enable
+    end function
+    integer(c_int) function hst_gpu_profile_count() bind(c, name='hst_gpu_profile_count')
+      import :: c_int
+    end function
+    integer(c_int) function hst_gpu_profile_entry(i, name, len, launches, ms, bytes) &
+      bind(c, name='hst_gpu_profile_entry')
+      import :: c_int, c_char, c_int64_t, c_double
+      integer(c_int), value :: i
+      character(kind=c_char), intent(out) :: name(*)
+      integer(c_int), value :: len
+      integer(c_int64_t), intent(out) :: launches
+      real(c_double), intent(out) :: ms, bytes
+    end function
   end interface
 
 contains

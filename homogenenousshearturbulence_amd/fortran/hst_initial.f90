@@ -6,11 +6,11 @@ module hst_initial
   use, intrinsic :: iso_c_binding
   use hst_params
   use hst_gpu_api
-  use hst_setup, only: deck_struct
+  use hst_setup, only: deck_struct, upload_velocity
 
   implicit none
   private
-  public :: generate_initial_field, uniform_from_key, upload_velocity
+  public :: generate_initial_field, uniform_from_key
 
 contains
 
@@ -22,14 +22,6 @@ contains
     call gpu_check(hst_initial_field(d, V), 'generate_initial_field')
     call upload_velocity()
   end subroutine generate_initial_field
-
-  ! the host copy of V to the device (`target update to(V)`)
-  subroutine upload_velocity()
-    integer(C_INT) :: c
-    do c = 1, 3
-      call gpu_check(hst_gpu_upload_field(c, V(:, :, :, c)), 'upload V')
-    end do
-  end subroutine upload_velocity
 
   ! A uniform deviate in [0, 1) from the seed and the mode indices (an
   ! fmix32-shaped avalanche masked to 32 bits).

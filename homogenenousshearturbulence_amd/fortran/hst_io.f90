@@ -1,24 +1,32 @@
-! Velocity files in the layout of the CPL code (the reference's module
-! hst_io): restart file Dati.cart.out and snapshots fields/field<n>.fld.  A
-! text header as io.cpl writes it, five raw doubles (time, S, S2, gamma_x,
-! gamma_y), the line "Vfield=", then the array (0..nx, -nz..nz, -2..ny+1) of
-! (u, w, v) complex, C order with the components innermost; the four ghost
-! rows are written as plain periodic copies.  One process writes and reads the
-! whole file with stream I/O.
+! Field files in the format of the CPL code (module hst_io of the reference,
+! src/hst_io.f90:5-33 for the format), read and written by this one process
+! with stream I/O.
+!
+!   file    = header, "Vfield=" LF, array       (velocity: Dati.cart.out, fields/field<n>.fld)
+!           = array                             (pressure: p_fields/pField<n>.fld)
+!   header  = text lines of "key=value" tokens separated by " " TAB, then five
+!             "name=" LF <8 raw bytes> LF records (time, S, S2, gamma_x, gamma_y)
+!   array   = A(c, r, iz, ix), c fastest: components (u, w, v) (one for the
+!             pressure), rows r = -2..ny+1 (the four ghost rows as plain
+!             periodic copies), iz = -nz..nz, ix = 0..nx, complex128
 module hst_io
 
   use, intrinsic :: iso_c_binding
   use hst_params
   use hst_gpu_api
+  use hst_setup, only: upload_velocity, download_velocity
   use hst_derivatives, only: s2_of, gamma_y_of
-  use hst_initial, only: generate_initial_field, upload_velocity
+  use hst_initial, only: generate_initial_field
 
   implicit none
   private
-  public :: restart_read, restart_write, make_output_dirs, download_velocity, field_write
+  public :: restart_read, restart_write, field_write, make_output_dirs
 
   character, parameter :: LF = achar(10), TAB = achar(9)
-  integer, parameter :: CPL_ORDER(3) = [1, 3, 2]     ! our component behind CPL's (u, v, w)
+  ! file component k holds our component FILE_COMP(k): the file's (u, v, w)
+  ! are its spanwise-last convention, (u, w, v) in ours
+  integer, parameter :: FILE_COMP(3) = [1, 3, 2]
+  integer, parameter :: HEAD_MAX = 4096
 
 contains
 
@@ -26,200 +34,247 @@ contains
     if (has_terminal) call execute_command_line('mkdir -p fields p_fields statistics statistics/spectra vtks')
   end subroutine make_output_dirs
 
-  ! the device velocity into the host copy V (`target update from(V)`)
-  subroutine download_velocity()
-    integer(C_INT) :: c
-    do c = 1, 3
-      call gpu_check(hst_gpu_download_field(c, V(:, :, :, c)), 'download V')
-    end do
-  end subroutine download_velocity
+  ! ------------------------------------------------------------------ reading
 
-  ! Reads filename into V, or generates the initial field when the file is
-  ! absent; the device copy is updated, the caller fills the ghosts
-  ! (restart_read, src/hst_io.f90:63-190).  A file of another resolution is
-  ! remapped: the Fourier modes present in both fields are copied (x and z are
-  ! mode indices), y is resampled by the nearest row of the file's uniform
-  ! grid.  Other alpha0/beta0 need restart_remap = 'mode_index', which then
-  ! repairs the discrete divergence on the device (project_restart).
+  ! Loads a velocity file into V and the device (the caller fills the ghost
+  ! rows), or, when there is no such file, the deck's initial field.  Another
+  ! resolution is accepted: the Fourier modes both grids have are taken (x and
+  ! z are mode indices), every other mode is zero, and each row of our y grid
+  ! takes the file row nearest to it on the file's uniform grid (the periodic
+  ! end wrapped).  Another alpha0 or beta0 needs restart_remap = 'mode_index',
+  ! and the discrete divergence is then repaired on the GPU.
   subroutine restart_read(filename)
     character(len=*), intent(in) :: filename
-    integer :: io, unit, p
-    character(len=4096) :: head
-    integer(C_INT) :: r_nx, r_ny, r_nz, source_ny, ix, iy, iz, src_iy
-    real(C_DOUBLE) :: r_alfa0, r_beta0, r_re, r_time, r_S, source_dy
-    complex(C_DOUBLE_COMPLEX), allocatable :: buf(:, :, :, :)
+    character(len=HEAD_MAX) :: head
+    integer :: unit, ios, body
+    integer(C_INT) :: fnx, fnz, fny
+    real(C_DOUBLE) :: f_alfa0, f_beta0, f_re, f_time, f_S
+    complex(C_DOUBLE_COMPLEX), allocatable :: A(:, :, :, :)
+    logical :: other_box
 
     open (newunit=unit, file=trim(filename), access='stream', form='unformatted', status='old', action='read', &
-          iostat=io)
-    if (io /= 0) then
-      if (has_terminal) print *, '   restart file '//trim(filename)//' not found'
+          iostat=ios)
+    if (ios /= 0) then
+      if (has_terminal) print '(A)', '   no field file '//trim(filename)//': starting from the initial condition'
       call generate_initial_field()
       return
     end if
-    if (has_terminal) print *, '   reading '//trim(filename)
     head = ''
-    read (unit, pos=1, iostat=io) head
-    p = index(head, 'Vfield='//LF)
-    if (p == 0) then
-      print *, 'ERROR: '//trim(filename)//' has no Vfield= line (not a CPL field file)'
+    read (unit, pos=1, iostat=ios) head
+    body = index(head, 'Vfield='//LF)
+    if (body == 0) then
+      close (unit)
+      print '(A)', ' ERROR: '//trim(filename)//': no "Vfield=" record, so not a velocity file of the CPL format'
       error stop 1
     end if
-    r_nx = int(header_value(head, 'nx='))
-    r_ny = int(header_value(head, 'ny='))
-    r_nz = int(header_value(head, 'nz='))
-    r_alfa0 = header_value(head, 'alpha0=')
-    r_beta0 = header_value(head, 'beta0=')
-    r_re = header_value(head, 'Re=')
-    source_ny = r_nz - 1
-    if (r_nx < 0 .or. r_ny < 0 .or. source_ny < 1) then
-      print *, 'ERROR: invalid mesh dimensions in '//trim(filename)
+    body = body + len('Vfield=') + 1            ! first byte of the array
+    ! the file's sizes: its nx, its spanwise modes ("ny"), its rows + 1 ("nz")
+    fnx = nint(header_number(head, 'nx'))
+    fnz = nint(header_number(head, 'ny'))
+    fny = nint(header_number(head, 'nz')) - 1
+    f_alfa0 = header_number(head, 'alpha0')
+    f_beta0 = header_number(head, 'beta0')
+    f_re = header_number(head, 'Re')
+    f_time = header_raw(unit, head, 'time')
+    f_S = header_raw(unit, head, 'S')
+    if (fnx < 0 .or. fnz < 0 .or. fny < 1) then
+      close (unit)
+      print '(A,3(1X,I0))', ' ERROR: '//trim(filename)//' declares impossible sizes (nx, nz, ny):', fnx, fnz, fny
       error stop 1
     end if
-    if (has_terminal .and. (r_nx /= nx .or. r_ny /= nz .or. source_ny /= ny)) then
-      print *, '   interpolating restart mesh: file (nx, nz, ny) = ', r_nx, r_ny, source_ny
-      print *, '                              deck (nx, nz, ny) = ', nx, nz, ny
+    if (has_terminal) then
+      print '(A)', '   loading '//trim(filename)
+      if (fnx /= nx .or. fnz /= nz .or. fny /= ny) &
+        print '(A,3(1X,I0),A,3(1X,I0),A)', '   resolution of the file (nx, nz, ny) =', fnx, fnz, fny, &
+        ', of the deck =', nx, nz, ny, ': modes and rows remapped'
+      if (abs(f_re - re) > 1.0d-9*re .or. f_S /= S) &
+        print '(A,2(1X,ES12.5))', '   (the file was written at another Re or S:', f_re, f_S
     end if
-    if (abs(r_alfa0 - alfa0) > 1.0d-4*max(abs(alfa0), 1.0d0) .or. abs(r_beta0 - beta0) > 1.0d-4*max(abs(beta0), 1.0d0)) &
-      then
-      if (trim(restart_remap) == 'mode_index') then
-        if (has_terminal) print *, '   restart_remap=mode_index: remapping signed Fourier indices and projecting' // &
-          ' on target wavenumbers'
-      else
-        print *, 'ERROR: restart alpha0/beta0 do not match the deck:', r_alfa0, r_beta0, alfa0, beta0
+    other_box = abs(f_alfa0 - alfa0) > 1.0d-4*max(abs(alfa0), 1.0d0) .or. &
+                abs(f_beta0 - beta0) > 1.0d-4*max(abs(beta0), 1.0d0)
+    if (other_box) then
+      if (trim(restart_remap) /= 'mode_index') then
+        close (unit)
+        print '(A,4(1X,ES14.7))', ' ERROR: the box of the file differs (alpha0, beta0 of file and deck):', &
+          f_alfa0, f_beta0, alfa0, beta0
+        print '(A)', '        restart_remap = ''mode_index'' maps the mode indices and repairs the divergence'
         error stop 1
       end if
+      if (has_terminal) print '(A)', '   another box: mode indices kept, divergence repaired on the new wavenumbers'
     end if
-    p = index(head, 'time='//LF)
-    read (unit, pos=p + 6) r_time
-    p = index(head, LF//'S='//LF)
-    read (unit, pos=p + 4) r_S
-    if (has_terminal .and. (abs(r_re - re) > 1.0d-9*re .or. r_S /= S)) print *, '   note: Re or S differ from the file'
-    if (time_from_restart) time = r_time
-    ! the file holds x = 0..r_nx, z = -r_ny..r_ny and the rows -2..source_ny+1
-    allocate (buf(3, -2:source_ny + 1, -r_ny:r_ny, 0:r_nx))
-    p = index(head, 'Vfield='//LF)
-    read (unit, pos=p + 8) buf
+    if (time_from_restart) time = f_time
+
+    allocate (A(3, -2:fny + 1, -fnz:fnz, 0:fnx))
+    read (unit, pos=body) A
     close (unit)
-    ! the CPL shear grid is uniform whatever the target's stretching: the
-    ! nearest row by coordinate, the end point wrapped (src/hst_io.f90:155-167)
-    source_dy = ly/real(source_ny, C_DOUBLE)
-    V = 0
-    do ix = 0, min(nx, r_nx)
-      do iz = max(-nz, -r_ny), min(nz, r_ny)
-        do iy = 0, ny - 1
-          src_iy = nint(modulo(y(iy), ly)/source_dy)
-          if (src_iy == source_ny) src_iy = 0
-          V(iy, iz, ix, CPL_ORDER) = buf(:, src_iy, iz, ix)
-        end do
-      end do
-    end do
-    deallocate (buf)
+    call take_rows(A, fnx, fnz, fny)
+    deallocate (A)
     call upload_velocity()
-    if (trim(restart_remap) == 'mode_index') then
-      call gpu_check(hst_gpu_project_divergence(time), 'project_restart')
+    if (other_box) then
+      call gpu_check(hst_gpu_project_divergence(time), 'divergence repair')
       call download_velocity()
     end if
   end subroutine restart_read
 
-  ! Writes the device velocity with the CPL header.
-  subroutine restart_write(filename)
-    character(len=*), intent(in) :: filename
-    character(len=:), allocatable :: head
-    complex(C_DOUBLE_COMPLEX), allocatable :: buf(:, :, :, :)
-    integer(C_INT) :: ix, iz
-    integer :: unit
-    call download_velocity()
-    head = 'nx='//str_i(nx)//' '//TAB//'ny='//str_i(nz)//' '//TAB//'nz='//str_i(ny + 1)// &
-           ' '//TAB//'alpha0='//str_r(alfa0)//' '//TAB//'beta0='//str_r(beta0)// &
-           ' '//TAB//'htcoeff='//str_r(merge(ystretch, -1.0d0, ystretch > 0.0d0))//' '//TAB//'Re='//str_r(re)// &
-           ' '//TAB//'Pr=0.71'//LF// &
-           'deltat='//str_r(deltat)//' '//TAB//'t_max='//str_r(t_max)//' '//TAB// &
-           'dt_field='//str_r(dt_field)//' '//TAB//'dt_save='//str_r(dt_save)//LF// &
-           't_field='//str_r(time)//LF// &
-           'meanpx=0 '//TAB//'meanflowx=0 '//TAB//'meanpy=0 '//TAB//'meanflowy=0'//LF// &
-           'time='//LF//raw8(time)//LF//'S='//LF//raw8(S)//LF//'S2='//LF//raw8(s2_of(time))//LF// &
-           'gamma_x='//LF//raw8(S*time)//LF//'gamma_y='//LF//raw8(gamma_y_of(time))//LF//'Vfield='//LF
-    allocate (buf(3, -2:ny + 1, -nz:nz, 0:nx))
-    do ix = 0, nx
-      do iz = -nz, nz
-        buf(:, 0:ny - 1, iz, ix) = transpose(V(0:ny - 1, iz, ix, CPL_ORDER))
-        buf(:, ny:ny + 1, iz, ix) = buf(:, 0:1, iz, ix)
-        buf(:, -2:-1, iz, ix) = buf(:, ny - 2:ny - 1, iz, ix)
+  ! V from the file array: for every row of our grid the nearest file row,
+  ! then the modes the two grids share.
+  subroutine take_rows(A, fnx, fnz, fny)
+    integer(C_INT), intent(in) :: fnx, fnz, fny
+    complex(C_DOUBLE_COMPLEX), intent(in) :: A(3, -2:fny + 1, -fnz:fnz, 0:fnx)
+    integer(C_INT) :: nearest(0:ny - 1), iy, k, mx, mz
+    real(C_DOUBLE) :: step
+    step = ly/real(fny, C_DOUBLE)
+    do iy = 0, ny - 1
+      nearest(iy) = modulo(nint(modulo(y(iy), ly)/step), fny)
+    end do
+    mx = min(nx, fnx)
+    mz = min(nz, fnz)
+    V = 0
+    do k = 1, 3
+      do iy = 0, ny - 1
+        V(iy, -mz:mz, 0:mx, FILE_COMP(k)) = A(k, nearest(iy), -mz:mz, 0:mx)
       end do
     end do
-    open (newunit=unit, file=trim(filename), access='stream', form='unformatted', status='replace', action='write')
-    write (unit) head
-    write (unit) buf
-    close (unit)
-    deallocate (buf)
-  end subroutine restart_write
+  end subroutine take_rows
 
-  ! One device field as a headerless CPL file of one component (field_write,
-  ! src/hst_io.f90:273-289): rows 0..ny-1, then ny, ny+1 as copies of 0, 1
-  ! and -2, -1 as copies of ny-2, ny-1, in the file order (-2..ny+1, -nz..nz,
-  ! 0..nx) with y fastest.
-  subroutine field_write(filename, fid)
-    character(len=*), intent(in) :: filename
-    integer(C_INT), intent(in) :: fid
-    complex(C_DOUBLE_COMPLEX), allocatable :: f(:, :, :), buf(:, :, :)
-    integer(C_INT) :: ix, iz
-    integer :: unit
-    allocate (f(-2:ny + 1, -nz:nz, 0:nx), buf(-2:ny + 1, -nz:nz, 0:nx))
-    call gpu_check(hst_gpu_download_field(fid, f), 'download field')
-    do ix = 0, nx
-      do iz = -nz, nz
-        buf(0:ny - 1, iz, ix) = f(0:ny - 1, iz, ix)
-        buf(ny:ny + 1, iz, ix) = f(0:1, iz, ix)
-        buf(-2:-1, iz, ix) = f(ny - 2:ny - 1, iz, ix)
-      end do
-    end do
-    open (newunit=unit, file=trim(filename), access='stream', form='unformatted', status='replace', action='write')
-    write (unit) buf
-    close (unit)
-    deallocate (f, buf)
-  end subroutine field_write
-
-  real(C_DOUBLE) function header_value(head, key)
+  ! The number after "key=" in the text part of the header: the key starts
+  ! the header or follows a separator, the value runs to the next one.
+  real(C_DOUBLE) function header_number(head, key) result(x)
     character(len=*), intent(in) :: head, key
     integer :: p, q
-    p = index(head, key)
-    if (p == 0) then
-      header_value = -huge(1.0d0)
-      return
-    end if
-    p = p + len(key)
-    q = p
-    do while (q <= len(head) .and. head(q:q) /= ' ' .and. head(q:q) /= TAB .and. head(q:q) /= LF)
-      q = q + 1
+    character :: before
+    x = -huge(1.0d0)
+    p = 0
+    do
+      q = index(head(p + 1:), key//'=')
+      if (q == 0) return
+      p = p + q
+      before = ' '
+      if (p > 1) before = head(p - 1:p - 1)
+      if (before == ' ' .or. before == TAB .or. before == LF) exit
     end do
-    read (head(p:q - 1), *) header_value
-  end function header_value
+    p = p + len(key) + 1
+    q = scan(head(p:), ' '//TAB//LF)
+    if (q <= 1) return
+    read (head(p:p + q - 2), *) x
+  end function header_number
 
-  function str_i(i) result(s)
+  ! The 8 raw bytes of the record "name=" LF <double> LF of the header.
+  real(C_DOUBLE) function header_raw(unit, head, name) result(x)
+    integer, intent(in) :: unit
+    character(len=*), intent(in) :: head, name
+    integer :: p
+    p = index(head, LF//name//'='//LF)
+    if (p == 0) then
+      print '(A)', ' ERROR: the field header has no '//name//'= record'
+      error stop 1
+    end if
+    read (unit, pos=p + len(name) + 3) x
+  end function header_raw
+
+  ! ------------------------------------------------------------------ writing
+
+  ! The velocity of the device state with the CPL header.
+  subroutine restart_write(filename)
+    character(len=*), intent(in) :: filename
+    call download_velocity()
+    call write_file(filename, cpl_header(), V, 3)
+  end subroutine restart_write
+
+  ! One array of the field shape as a headerless one-component CPL file.
+  subroutine field_write(filename, field)
+    character(len=*), intent(in) :: filename
+    complex(C_DOUBLE_COMPLEX), intent(in) :: field(-2:ny + 1, -nz:nz, 0:nx)
+    call write_file(filename, '', field, 1)
+  end subroutine field_write
+
+  ! head, then the array of ncomp components of f (our layout f(iy, iz, ix, c))
+  subroutine write_file(filename, head, f, ncomp)
+    character(len=*), intent(in) :: filename, head
+    integer, intent(in) :: ncomp
+    complex(C_DOUBLE_COMPLEX), intent(in) :: f(-2:ny + 1, -nz:nz, 0:nx, ncomp)
+    complex(C_DOUBLE_COMPLEX), allocatable :: A(:, :, :, :)
+    integer :: unit, k, ix, iz, c
+    allocate (A(ncomp, -2:ny + 1, -nz:nz, 0:nx))
+    do k = 1, ncomp
+      c = k
+      if (ncomp == 3) c = FILE_COMP(k)
+      do ix = 0, nx
+        do iz = -nz, nz
+          A(k, 0:ny - 1, iz, ix) = f(0:ny - 1, iz, ix, c)
+        end do
+      end do
+    end do
+    ! the ghost rows of the file: plain periodic copies of the interior
+    A(:, ny:ny + 1, :, :) = A(:, 0:1, :, :)
+    A(:, -2:-1, :, :) = A(:, ny - 2:ny - 1, :, :)
+    if (.not. has_terminal) return
+    open (newunit=unit, file=trim(filename), access='stream', form='unformatted', status='replace', action='write')
+    if (len(head) > 0) write (unit) head
+    write (unit) A
+    close (unit)
+    deallocate (A)
+  end subroutine write_file
+
+  ! The header of a velocity file at the current time.
+  function cpl_header() result(h)
+    character(len=:), allocatable :: h
+    real(C_DOUBLE) :: htcoeff
+    htcoeff = -1.0d0
+    if (ystretch > 0.0d0) htcoeff = ystretch
+    h = fields([character(len=40) :: 'nx='//int_text(nx), 'ny='//int_text(nz), 'nz='//int_text(ny + 1), &
+                'alpha0='//real_text(alfa0), 'beta0='//real_text(beta0), 'htcoeff='//real_text(htcoeff), &
+                'Re='//real_text(re)], 'Pr=0.71')
+    h = h//fields([character(len=40) :: 'deltat='//real_text(deltat), 't_max='//real_text(t_max), &
+                   'dt_field='//real_text(dt_field)], 'dt_save='//real_text(dt_save))
+    h = h//'t_field='//real_text(time)//LF
+    h = h//fields([character(len=40) :: 'meanpx=0', 'meanflowx=0', 'meanpy=0'], 'meanflowy=0')
+    h = h//raw_record('time', time)//raw_record('S', S)//raw_record('S2', s2_of(time))// &
+        raw_record('gamma_x', S*time)//raw_record('gamma_y', gamma_y_of(time))//'Vfield='//LF
+  end function cpl_header
+
+  ! one header line: the tokens, each followed by " " TAB, then the last and LF
+  function fields(tokens, last) result(line)
+    character(len=*), intent(in) :: tokens(:), last
+    character(len=:), allocatable :: line
+    integer :: i
+    line = ''
+    do i = 1, size(tokens)
+      line = line//trim(tokens(i))//' '//TAB
+    end do
+    line = line//last//LF
+  end function fields
+
+  function raw_record(name, x) result(r)
+    character(len=*), intent(in) :: name
+    real(C_DOUBLE), intent(in) :: x
+    character(len=:), allocatable :: r
+    character(len=8) :: bytes
+    bytes = transfer(x, bytes)
+    r = name//'='//LF//bytes//LF
+  end function raw_record
+
+  function int_text(i) result(s)
     integer(C_INT), intent(in) :: i
     character(len=:), allocatable :: s
-    character(len=32) :: t
+    character(len=16) :: t
     write (t, '(I0)') i
     s = trim(t)
-  end function str_i
+  end function int_text
 
-  ! G0.17 with the leading zero of a fraction written out (0.1..., -0.1...)
-  function str_r(x) result(s)
+  ! the list-directed G0.17 text of x that the CPL tools expect, with the
+  ! leading zero of a number below one written out (0.1..., -0.1...)
+  function real_text(x) result(s)
     real(C_DOUBLE), intent(in) :: x
     character(len=:), allocatable :: s
     character(len=32) :: t
     write (t, '(G0.17)') x
     s = trim(adjustl(t))
-    if (s(1:1) == '.') s = '0'//s
-    if (len(s) > 1) then
+    if (s(1:1) == '.') then
+      s = '0'//s
+    else if (len(s) > 1) then
       if (s(1:2) == '-.') s = '-0'//s(2:)
     end if
-  end function str_r
-
-  function raw8(x) result(s)
-    real(C_DOUBLE), intent(in) :: x
-    character(len=8) :: s
-    s = transfer(x, s)
-  end function raw8
+  end function real_text
 
 end module hst_io

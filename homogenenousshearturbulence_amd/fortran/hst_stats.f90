@@ -9,6 +9,7 @@ module hst_stats
   use hst_params
   use hst_gpu_api
   use hst_derivatives, only: s2_of, gamma_y_of
+  use hst_setup, only: state_in, S_V
 
   implicit none
   private
@@ -70,6 +71,7 @@ contains
   ! energy, diss, uv, vw, uu, vv, ww, uw (box means) and the profile integrals mfx, mfz
   subroutine outstats()
     real(C_DOUBLE) :: o(10), e(8)
+    call state_in(S_V)
     call gpu_check(hst_gpu_outstats(time, o), 'outstats')
     if (has_terminal) then
       write (*, '(F12.5,2X,ES11.4,2X,F8.4,4(2X,ES13.6))') time, deltat, cfl*deltat, o(1), o(2), o(3), o(4)

@@ -35,7 +35,8 @@ EXPORTS = (
     "hst_gpu_timestep", "hst_gpu_outstats", "hst_gpu_set_chunk_rows", "hst_gpu_memory", "hst_gpu_profile",
     "hst_gpu_profile_count", "hst_gpu_profile_entry", "hst_gpu_pressure", "hst_gpu_spectrum",
     "hst_gpu_stokes_apply", "hst_gpu_damping_mask", "hst_gpu_outstats_extra", "hst_gpu_project_divergence",
-    "hst_gpu_bandwidth_probe",
+    "hst_gpu_bandwidth_probe", "hst_gpu_fft_z", "hst_gpu_fft_x",
+    "hst_gpu_products", "hst_gpu_set_products",
 )
 
 
@@ -91,6 +92,10 @@ def lib() -> ctypes.CDLL:
         "hst_gpu_damping_mask": [],
         "hst_gpu_outstats_extra": [dp],
         "hst_gpu_bandwidth_probe": [d, i, dp],
+        "hst_gpu_fft_z": [vp, ctypes.c_int64, i],
+        "hst_gpu_fft_x": [vp, ctypes.c_int64, i],
+        "hst_gpu_products": [i, ctypes.c_int64, vp, vp],
+        "hst_gpu_set_products": [i, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

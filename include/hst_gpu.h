@@ -134,6 +134,31 @@ int hst_gpu_product_to_field(int p, int field);
  * -> (times factor) -> spectral, left as products 1..3. */
 int hst_gpu_roundtrip(void);
 
+/* ---- per-field transforms on the reference's host buffers (module hst_fft's
+ * FFT, IFT, RFT, HFT, src/hst_fft.f90:181-237, and build_products,
+ * src/hst_transforms.f90:97-121), FP64 on the device whatever the precision of
+ * the state; each call moves its lines to the device and back.  For callers
+ * that run the transforms field by field; the time step does not use them. */
+
+/* IFT (sign +1, exp(+i)) or FFT (sign -1) of nlines z lines of nzd complex,
+ * contiguous (VVdz(:, :, :, m): nlines = (nx+1)(ny+4)), in place, unnormalised. */
+int hst_gpu_fft_z(double *lines, int64_t nlines, int sign);
+/* RFT (dir +1, c2r: modes 0..nxd -> 2 nxd real points, the imaginary parts of
+ * modes 0 and nxd ignored, the two spare reals of a line set to 0) or HFT
+ * (dir -1, r2c) of nlines x lines stored as nxd+1 complex (VVdx, VVdp: the
+ * padded in-place layout), in place, unnormalised. */
+int hst_gpu_fft_x(double *lines, int64_t nlines, int dir);
+/* build_products(g): vel = rVVdx(:, :, :, 1:3), prod = products(:, :, :, 1:3),
+ * each field nlines x lines of 2(nxd+1) reals; group 1: uu, vv, ww, 2: uv,
+ * vw, uw, times factor = 1/(2 nxd nzd), on the 2 nxd points of a line (the
+ * spare reals of prod are left as they are). */
+int hst_gpu_products(int g, int64_t nlines, const double *vel, double *prod);
+/* The product spectra VVdz(nzd, nx+1, ny+4, 1:3) of group g (as
+ * products_to_spectral leaves them) into the device state, where
+ * hst_gpu_buildrhs(ode, g) reads them: the kept modes iz = -nz..nz
+ * (VVdz(izd(iz) + 1, ...)) of every row. */
+int hst_gpu_set_products(int g, const double *vvdz);
+
 /* buildrhs_prepare (src/hst_equations.f90:92-141) at `time` with step deltat. */
 int hst_gpu_buildrhs_prepare(const double ode[3], double deltat, double time);
 /* buildrhs(ODE, g) (src/hst_equations.f90:151-213) from the product spectra of group g. */

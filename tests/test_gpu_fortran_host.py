@@ -142,7 +142,7 @@ def test_restart_remap(tmp_path, oracle_built, remap):
     assert (d.nx, d.ny, d.nz, d.ystretch) == (21, 45, 11, 1.2)
     r = subprocess.run([HST, "hst.in"], cwd=tmp_path, capture_output=True, text=True, timeout=150)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-    assert "interpolating restart mesh" in r.stdout
+    assert "modes and rows remapped" in r.stdout
     t, V, sizes = cplio.read_field(str(tmp_path / "Dati.cart.out"))
     tf, Vf, sf = cplio.read_field(src)
     assert t == tf and sizes == {"nx": 21, "nz": 11, "ny": 45}

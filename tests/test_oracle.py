@@ -523,3 +523,62 @@ def test_adaptive_step_keeps_cfl_at_cflmax(O):
     assert len(set(o.dts)) == 6            # the step changes with the field
     assert abs(o.time - sum(o.dts)) <= 1e-14 * o.time
     assert o.deltat == 0.8 / o.cfl
+
+
+def _box_energy(V, d):
+    """q2 = <u_i u_i>: sum over the modes (ix > 0 twice) of the interior rows / ny."""
+    w = np.full(d.nx + 1, 2.0)
+    w[0] = 1.0
+    return float((w[:, None, None] * (np.abs(V[:, :, :, 2:d.ny + 2]) ** 2).sum(axis=0)).sum() / d.ny)
+
+
+def _product_rates(O, d, dt, scale=None):
+    """(dq2/dt)/q2 over one RK3 step with one product kept (1..6) or all
+    (0), after one unmasked step from the seeded field; scale[p] multiplies
+    product p when given (a planted mistake)."""
+    import ctypes
+
+    L = O.lib()
+    rates = np.zeros(7)
+    for keep in range(7):
+        o = O.Oracle(d)
+        o.time = 0.0
+        o.initial_field()
+        o.fill_all_ghosts(0.0)
+        o.timestep(dt)  # the first step projects onto the discretely solenoidal space
+        q0 = _box_energy(o.V, d)
+        t = o.time
+        for ode in O.RK_RAI:
+            c = (ctypes.c_double * 3)(*ode)
+            o.nonlinear(0)
+            L.hsto_buildrhs_prepare(o.h, c, dt, t)
+            P = o.products()
+            for p in range(6):
+                if keep and p != keep - 1:
+                    P[p][...] = 0.0
+                elif scale is not None:
+                    P[p][...] *= scale[p]
+            L.hsto_buildrhs(o.h, c, 1)
+            L.hsto_buildrhs(o.h, c, 2)
+            dts = 2.0 / ode[0] * dt
+            L.hsto_shear_shift(o.h, dts, t)
+            t = t + dts
+            L.hsto_linsolve(o.h, ode[0] / dt, t)
+        rates[keep] = (_box_energy(o.V, d) - q0) / dt / q0
+    return rates
+
+
+def test_conservation_of_the_products(O):
+    """tests/test_conservation.f90:35-94 on the oracle: with S = 0 and
+    nu = 1e-12, uu, vv, ww alone inject no energy (u d_x(u^2) integrates to
+    zero), uv + vw + uw together inject none, and neither do all six: each
+    rate below 1e-2 (discretely O(dt) plus the compact scheme's small
+    non-conservation).  The cross products alone are not small, so doubling
+    uv (a planted mistake in one product's term) breaks the sum."""
+    d = dataclasses.replace(read_input(deck_path("pressure")), S=0.0, re=1e12)
+    r = _product_rates(O, d, 1e-4)
+    assert np.abs(r[:4]).max() <= 1e-2, r
+    assert abs(r[4] + r[5] + r[6]) <= 1e-2, r
+    assert np.abs(r[4:]).max() > 0.1, r  # the cross terms carry energy between components
+    bad = _product_rates(O, d, 1e-4, scale=[1, 1, 1, 2, 1, 1])
+    assert abs(bad[0]) > 1e-2 and abs(2 * r[4] + r[5] + r[6]) > 1e-2, (bad, r)
