@@ -110,6 +110,18 @@ struct State {
 static State S;
 static int g_chunk_request = 0;
 
+// ------------------------------------------------------------------ the no-allocation workspace
+
+// The workspace of hst_gpu_line_solve_noalloc: a file-scope device array, so
+// that the entry point allocates nothing.  Every device the library runs on
+// holds one copy until the process ends.  Host code never passes the array's
+// name to a kernel (on the host it only stands for the symbol): it takes the
+// device address once per device (scratch_attach) and passes that.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)HST_NOALLOC_MAX_BYTES;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+static void *g_scratch_dev[MAX_DEVICES] = {nullptr};
+
 static bool field_f32(int f) { return S.mixed && (f == HST_OLD1 || f == HST_OLD2); }
 
 static int fail(const std::string &what, int rc = 1) {
@@ -126,6 +138,25 @@ static int hip_fail(const char *where, hipError_t e) {
     hipError_t _e = (call);                                   \
     if (_e != hipSuccess) return hip_fail(#call, _e);         \
   } while (0)
+
+// The device address of the current device's copy of scratch, taken once
+// per device with that device current (at hst_gpu_init, or at the first
+// call on a device that init did not run on).
+static int scratch_attach(void **ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(std::string("hipGetDevice: ") + hipGetErrorString(e), (int)e);
+  if (dev < 0 || dev >= MAX_DEVICES) return fail("device ordinal out of range for the workspace table");
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e), e ? (int)e : 1);
+    g_scratch_dev[dev] = p;
+  }
+  if (ptr) *ptr = g_scratch_dev[dev];
+  return 0;
+}
 
 static hipEvent_t take_event() {
   if (!S.pool.empty()) {
@@ -794,6 +825,7 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
   HIPCHK(hipMalloc(&S.d_part, S.part_cap * sizeof(double)));
   HIPCHK(hipHostMalloc((void **)&S.h_part, S.part_cap * sizeof(double), hipHostMallocDefault));
   if (make_fft_tables(D, S.mixed, &S.tables)) return fail("twiddle tables");
+  if (int rc = scratch_attach(nullptr)) return rc;
   S.bytes_held = held;
   S.live = true;
   return 0;
@@ -1066,6 +1098,49 @@ int hst_gpu_line_solve(int kind, double lambda, int src, int dst, double shift_x
   return run("penta", 4.0 * bytes_field(), [&] {
     return launch_penta(S.D, 1, kinds, lambda, s, d, S.pws, nullptr, S.d_der, shift_x, shift_z, S.stream);
   });
+}
+
+// [p, p + bytes) inside one device allocation of this process
+static int device_range(const void *p, size_t bytes, const char *what) {
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  hipError_t e = hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p);
+  if (e != hipSuccess || !base) {
+    // the failed query leaves its status as the thread's last HIP error: take
+    // it back, or the caller's next launch check would report it as its own
+    (void)hipGetLastError();
+    return fail(std::string(what) + " is not device memory of this process (hipMemGetAddressRange: " +
+                hipGetErrorString(e) + ")");
+  }
+  if ((const char *)p + bytes > (const char *)base + size)
+    return fail(std::string(what) + ": the allocation ends before the ny+4 rows of ncols columns");
+  return 0;
+}
+
+int hst_gpu_line_solve_noalloc(int kind, double lambda, int col0, int ncols, const double *src, double *dst,
+                               double shift_x, double shift_z) {
+  if (int rc = check_live_keep()) return rc;
+  if (kind < HST_KIND_D2V || kind > HST_KIND_DY) return fail("line_solve_noalloc: unknown system kind");
+  if (ncols < 1 || col0 < 0 || col0 > S.D.ncol - ncols)
+    return fail("line_solve_noalloc: columns " + std::to_string(col0) + " + " + std::to_string(ncols) +
+                " outside 0.." + std::to_string(S.D.ncol - 1));
+  if (!src || !dst) return fail("line_solve_noalloc: src or dst is NULL");
+  const size_t nbytes = (size_t)S.D.NYg * (size_t)ncols * sizeof(double2);
+  if ((const char *)src < (const char *)dst + nbytes && (const char *)dst < (const char *)src + nbytes)
+    return fail("line_solve_noalloc: src and dst overlap");
+  const size_t need = line_cols_ws_bytes(S.D, kind, ncols);
+  if (need > NOALLOC_SCRATCH_BYTES)
+    return fail("line_solve_noalloc: " + std::to_string(ncols) + " columns need " + std::to_string(need) +
+                " bytes of workspace, above the fixed " + std::to_string(NOALLOC_SCRATCH_BYTES) +
+                " (HST_NOALLOC_MAX_BYTES): solve fewer columns per call");
+  if (int rc = device_range(src, nbytes, "line_solve_noalloc: src")) return rc;
+  if (int rc = device_range(dst, nbytes, "line_solve_noalloc: dst")) return rc;
+  void *ws = nullptr;
+  if (int rc = scratch_attach(&ws)) return rc;
+  const int rc = launch_line_cols(S.D, kind, lambda, col0, ncols, (const double2 *)src, (double2 *)dst, ws, S.d_f0,
+                                  S.fc, S.d_der, shift_x, shift_z, S.stream);
+  if (rc) return hip_fail("line_solve_noalloc", (hipError_t)rc);
+  return 0;
 }
 
 int hst_gpu_linsolve(double lambda, double time) {

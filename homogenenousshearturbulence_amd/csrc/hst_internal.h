@@ -125,6 +125,12 @@ int launch_rfft_lines(int m, int dir, double2 *data, int64_t nlines, const void 
                       hipStream_t s);
 int launch_products(const double *vel, double *prod, int64_t plane, int lw, int used, int g, double factor,
                     hipStream_t s);
+// line_solve over a range of columns held in arrays of their own ([NYg][ncols]),
+// workspace ws of line_cols_ws_bytes(D, kind, ncols) bytes
+size_t line_cols_ws_bytes(const Dims &D, int kind, int ncols);
+int launch_line_cols(const Dims &D, int kind, double lambda, int col0, int ncols, const double2 *src, double2 *dst,
+                     void *ws, const double *f0rows, const D0Const &fc, const double *der, double sx, double sz,
+                     hipStream_t s);
 
 // The exact shift of q[0..nq) (rhs1, rhs2, oldrhs1, oldrhs2): D0 solve at the
 // old shifts, node phase, ghosts at the new shifts, D0 re-weight.

@@ -881,6 +881,89 @@ int launch_d0solve(const Dims &D, int rhs_mode, int out_mode, const double2 *src
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ line solves of a column range
+
+// line_solve over the columns col0 .. col0 + ncols - 1 only, on arrays that
+// hold just those columns ([ny + 4 rows][ncols], row stride ncols): the
+// wavenumbers and the wrap phase are those of the global column col0 + l,
+// everything else (the order of the elimination, the border system, the
+// back substitution) is that of k_penta (untabled) and k_d0solve (OUT 0), so
+// a column comes out with the same bits as through the whole-field solves.
+// ws: the workspace of ncols columns (U1, U2, B1, B2, XR of the untabled
+// sweep; XR only for D0 and DY).
+template <int K>
+__global__ void __launch_bounds__(256)
+    k_line_cols(Dims D, int col0, int ncols, double lam, const double *__restrict__ der,
+                const double *__restrict__ f0, D0Const fc, const double2 *__restrict__ src,
+                double2 *__restrict__ dst, PentaWS ws, double sx, double sz) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= ncols) return;
+  int ix, iz;
+  col_ix_iz(D, col0 + l, ix, iz);
+  Dims Ds = D;
+  Ds.ncol = ncols;  // the row stride of src, dst and ws
+  if constexpr (K == KD0 || K == KDY) {
+    const size_t nc = ncols;
+    const int m = D.ny;
+    C x1, x2, sx0, sx1, r25, r27;
+    d0_forward<K == KDY ? 1 : 0>(Ds, f0, der, src, ws.XR, l, x1, x2, sx0, sx1, r25, r27);
+    const C ph = cexpi(-(D.alfa0 * ix * sx + D.beta0 * iz * sz));
+    C b1, b2;
+    d0_border(fc, ph, x1, x2, sx0, sx1, r25, r27, b1, b2);
+    const C cph = conj(ph);
+    const C yb1 = cph * b1, yb2 = cph * b2;
+    st(dst + (size_t)(m - 2 + 2) * nc + l, b1);
+    st(dst + (size_t)(m - 1 + 2) * nc + l, b2);
+    C xk1 = b1, xk2 = b2;
+    rows_descending<PF_BWD, C>(
+        m - 2, [&](int i) { return ld(ws.XR + (size_t)i * nc + l); },
+        [&](int i, const C &xr) {
+          const C xk = xr - F0(i, 5) * yb1 - F0(i, 6) * yb2 - F0(i, 3) * xk1 - F0(i, 4) * xk2;
+          st(dst + (size_t)(i + 2) * nc + l, xk);
+          xk2 = xk1;
+          xk1 = xk;
+        });
+  } else {
+    penta_line<K, false>(Ds, l, ix, iz, 0, lam, der, src, dst, ws, nullptr, sx, sz);
+  }
+}
+
+size_t line_cols_ws_bytes(const Dims &D, int kind, int ncols) {
+  const size_t rows = (size_t)(D.ny - 2) * (size_t)ncols;
+  return rows * ((kind == KD0 || kind == KDY) ? 16 : 48);
+}
+
+int launch_line_cols(const Dims &D, int kind, double lambda, int col0, int ncols, const double2 *src, double2 *dst,
+                     void *ws, const double *f0rows, const D0Const &fc, const double *der, double sx, double sz,
+                     hipStream_t s) {
+  const size_t rows = (size_t)(D.ny - 2) * (size_t)ncols;
+  char *w = (char *)ws;
+  PentaWS pw;
+  if (kind == KD0 || kind == KDY) {
+    pw.U1 = pw.U2 = pw.B1 = pw.B2 = nullptr;
+    pw.XR = (double2 *)w;
+  } else {
+    pw.XR = (double2 *)w;  // 16-byte entries first: every array stays 16-byte aligned
+    pw.U1 = (double *)(w + rows * 16);
+    pw.U2 = (double *)(w + rows * 24);
+    pw.B1 = (double *)(w + rows * 32);
+    pw.B2 = (double *)(w + rows * 40);
+  }
+  dim3 grid((ncols + 255) / 256);
+#define L(K) hipLaunchKernelGGL((k_line_cols<K>), grid, dim3(256), 0, s, D, col0, ncols, lambda, der, f0rows, fc, src, \
+                                dst, pw, sx, sz)
+  switch (kind) {
+    case KD2V: L(KD2V); break;
+    case KETA: L(KETA); break;
+    case KPOISSON: L(KPOISSON); break;
+    case KD0: L(KD0); break;
+    case KDY: L(KDY); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef L
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ exact shift
 
 // shift_unweighted (src/hst_equations.f90:263-293) of one array per lane:

@@ -35,7 +35,7 @@ EXPORTS = (
     "hst_gpu_timestep", "hst_gpu_outstats", "hst_gpu_set_chunk_rows", "hst_gpu_memory", "hst_gpu_profile",
     "hst_gpu_profile_count", "hst_gpu_profile_entry", "hst_gpu_pressure", "hst_gpu_spectrum",
     "hst_gpu_stokes_apply", "hst_gpu_damping_mask", "hst_gpu_outstats_extra", "hst_gpu_project_divergence",
-    "hst_gpu_bandwidth_probe", "hst_gpu_fft_z", "hst_gpu_fft_x",
+    "hst_gpu_bandwidth_probe", "hst_gpu_line_solve_noalloc", "hst_gpu_fft_z", "hst_gpu_fft_x",
     "hst_gpu_products", "hst_gpu_set_products",
 )
 
@@ -92,6 +92,7 @@ def lib() -> ctypes.CDLL:
         "hst_gpu_damping_mask": [],
         "hst_gpu_outstats_extra": [dp],
         "hst_gpu_bandwidth_probe": [d, i, dp],
+        "hst_gpu_line_solve_noalloc": [i, d, i, i, vp, vp, d, d],
         "hst_gpu_fft_z": [vp, ctypes.c_int64, i],
         "hst_gpu_fft_x": [vp, ctypes.c_int64, i],
         "hst_gpu_products": [i, ctypes.c_int64, vp, vp],
@@ -271,6 +272,15 @@ class Solver:
 
     def line_solve(self, kind: int, lam: float, src: int, dst: int, sx: float, sz: float) -> None:
         _check(lib().hst_gpu_line_solve(kind, lam, src, dst, sx, sz), "hst_gpu_line_solve")
+
+    def line_solve_noalloc(self, kind: int, lam: float, col0: int, ncols: int, src_ptr: int, dst_ptr: int,
+                           sx: float, sz: float) -> None:
+        """line_solve of the columns col0 .. col0+ncols-1 between caller-owned
+        device arrays complex128 [ny+4][ncols] (device addresses), with the
+        library's fixed workspace: no device allocation.  Enqueued on the
+        library's stream; sync() waits for it."""
+        _check(lib().hst_gpu_line_solve_noalloc(kind, lam, col0, ncols, src_ptr, dst_ptr, sx, sz),
+               "hst_gpu_line_solve_noalloc")
 
     def linsolve(self, lam: float, time: float) -> None:
         _check(lib().hst_gpu_linsolve(lam, time), "hst_gpu_linsolve")

@@ -168,6 +168,26 @@ int hst_gpu_buildrhs(const double ode[3], int g);
 int hst_gpu_shear_shift(double dt_sub, double time);
 /* line_solve(sys, lambda, src, dst, shift_x, shift_z) (src/hst_linsolve.f90:147-233). */
 int hst_gpu_line_solve(int kind, double lambda, int src, int dst, double shift_x, double shift_z);
+/* The line solve of hst_gpu_line_solve on caller-owned device arrays, with no
+ * allocation: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight solver loops,
+ * harnesses that forbid hipMalloc).  Solves the lines of the columns
+ * col0 .. col0 + ncols - 1 (column order (iz + nz) + (2nz + 1) ix): src and
+ * dst are device arrays of complex128 [ny + 4 rows][ncols] (row stride ncols,
+ * rows iy = -2 .. ny + 1) holding just those columns; the interior rows of dst
+ * receive the solution (the ghost rows are not written; KIND_DY reads the
+ * ghost rows of src).  The workspace is a fixed device array of
+ * HST_NOALLOC_MAX_BYTES inside the library (per device, held until the
+ * process ends): ncols * (ny - 2) * 48 bytes for D2V, ETA, POISSON and
+ * * 16 bytes for D0, DY must fit in it, else the call fails (solve fewer
+ * columns per call).  Needs hst_gpu_init (the grid and its tables); calls no
+ * hipMalloc or hipFree.  Enqueued on the library's stream (hst_gpu_sync
+ * waits for it).  Every call on a device shares that workspace: calls that
+ * overlap in time on one device race.  Nothing in the workspace is read
+ * before the call writes it. */
+#define HST_NOALLOC_MAX_BYTES (128u * 1024u * 1024u)
+int hst_gpu_line_solve_noalloc(int kind, double lambda, int col0, int ncols, const double *src, double *dst,
+                               double shift_x, double shift_z);
 /* linsolve(lambda) at `time` (src/hst_equations.f90:298-326): solves, ghosts, u and w. */
 int hst_gpu_linsolve(double lambda, double time);
 

@@ -83,3 +83,12 @@ def test_benchmark_decks_have_unit_rms_velocity():
         d = read_input(os.path.join(ROOT, "homogenenousshearturbulence_amd", "decks", name + ".in"))
         assert abs(hst.initial_energy(d) - 3.0) < 1e-12
         assert d.alfa0 == 1.0 and d.beta0 == 1.0 and d.cflmax == 0.0
+
+
+def test_noalloc_line_solve_needs_init():
+    """The line solve that allocates nothing reads the grid and tables of an
+    initialised state: without hst_gpu_init it fails cleanly, no launch."""
+    L = hst.lib()
+    rc = L.hst_gpu_line_solve_noalloc(1, 1.0, 0, 1, None, None, 0.0, 0.0)
+    assert rc != 0
+    assert "hst_gpu_init" in hst.last_error()
