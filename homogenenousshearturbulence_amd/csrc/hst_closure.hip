@@ -160,8 +160,8 @@ __global__ void __launch_bounds__(XG * fft_threads(M))
   }
 }
 
-// x forward real transform (times factor) of six physical fields into
-// Hp[p][row][zp][ix], ix <= nx, the layout the z forward transform reads.
+// x forward real transform (times factor) of six physical fields into the
+// product buffer in the tiled layout the z forward transform reads (hoff).
 template <int M>
 __global__ void __launch_bounds__(XG * fft_threads(M))
     k_cxr2c(const double *__restrict__ phys, Cx<double> *__restrict__ Hp, Dims D,
@@ -175,12 +175,12 @@ __global__ void __launch_bounds__(XG * fft_threads(M))
   for (int m = t; m < M; m += T) vl[m] = Cx<double>{in[2 * m] * D.factor, in[2 * m + 1] * D.factor};
   __syncthreads();
   fft_line<double, M, T, -1>(vl, t, twx);
-  Cx<double> *dst = Hp + (((size_t)p * D.ny + rowc) * D.nzd + zp) * D.NXp;
+  Cx<double> *dst = Hp + ((size_t)p * D.ny + rowc) * D.nzd * D.NXp;
   for (int k = t; k <= nx; k += T) {
     const Cx<double> Z = vl[k], Zc = conj(vl[(M - k) % M]);
     const Cx<double> e = twh[k];
     const Cx<double> s = (Z + Zc) * 0.5, d = (Z - Zc) * 0.5;
-    dst[k] = s - mul_i(e * d);
+    dst[hoff<double>(D.nzd, D.NXp, zp, k)] = s - mul_i(e * d);
   }
 }
 

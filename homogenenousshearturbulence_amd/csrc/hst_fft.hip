@@ -119,25 +119,6 @@ struct XCfg {
 
 // ------------------------------------------------------------------ transform buffers
 
-// Layout of one (component, row) plane of Hv / Hp (nzd z points x NXp x modes).
-// Hv is tiled by the z inverse transform's modes per block (4 in FP32, 2 in
-// FP64), each tile z-major ([ix / HT][zp][ix % HT]): a block of that transform
-// writes one contiguous range instead of HT-mode pieces of every z row, and
-// the x pass reads its line in HT-mode pieces from lines that neighbouring
-// blocks share through L2.  Hp is [zp][ix] (tiling it as well made the x
-// pass's stores slower than the z forward transform's loads gained).
-// B: 0 for Hv, 1 for Hp.
-template <typename R>
-constexpr int htile() { return sizeof(R) == 4 ? 4 : 2; }
-template <typename R, int B>
-__device__ __forceinline__ size_t hoff(int NZD, int NXp, int zp, int ix) {
-  constexpr int HT = htile<R>();
-  if constexpr (HT > 0 && B == 0)
-    return (size_t)(ix / HT) * ((size_t)NZD * HT) + (size_t)zp * HT + ix % HT;
-  else
-    return (size_t)zp * NXp + ix;
-}
-
 // ------------------------------------------------------------------ XCD-aware block order
 
 // Workgroups are dealt to the 8 XCDs round-robin by their linear id, and each
@@ -203,7 +184,7 @@ __global__ void __launch_bounds__((ZCfg<R, NZD, false>::NT))
     Cx<R> *dst = Hv + ((size_t)cc * nrows + rowc) * NZD * D.NXp;
     for (int e = threadIdx.x; e < G * NZD; e += blockDim.x) {
       const int j = e % G, zp = e / G;
-      if (ix0 + j < D.NXm) dst[hoff<R, 0>(NZD, D.NXp, zp, ix0 + j)] = lds[j * LS + zp];
+      if (ix0 + j < D.NXm) dst[hoff<R>(NZD, D.NXp, zp, ix0 + j)] = lds[j * LS + zp];
     }
   } else {
     const int ix0 = tile * HG;
@@ -247,7 +228,7 @@ __global__ void __launch_bounds__((ZCfg<R, NZD, false>::NT))
       Cx<R> *dst = Hv + ((size_t)(2 * h) * nrows + rowc) * NZD * D.NXp;
       for (int e = threadIdx.x; e < HG * NZD; e += blockDim.x) {
         const int j = e % HG, zp = e / HG;
-        if (ix0 + j < D.NXm) dst[hoff<R, 0>(NZD, D.NXp, zp, ix0 + j)] = lds[(h * HG + j) * LS + zp];
+        if (ix0 + j < D.NXm) dst[hoff<R>(NZD, D.NXp, zp, ix0 + j)] = lds[(h * HG + j) * LS + zp];
       }
     }
   }
@@ -271,7 +252,7 @@ __global__ void __launch_bounds__((ZCfg<R, NZD, true>::NT))
   const Cx<R> *src = Hp + ((size_t)p * nrows + rowc) * NZD * D.NXp;
   for (int e = threadIdx.x; e < G * NZD; e += blockDim.x) {
     const int j = e % G, zp = e / G;
-    lds[j * LS + zp] = (ix0 + j < D.NXm) ? src[hoff<R, 1>(NZD, D.NXp, zp, ix0 + j)] : Cx<R>{(R)0, (R)0};
+    lds[j * LS + zp] = (ix0 + j < D.NXm) ? src[hoff<R>(NZD, D.NXp, zp, ix0 + j)] : Cx<R>{(R)0, (R)0};
   }
   __syncthreads();
   const int g = threadIdx.x / T, t = threadIdx.x % T;
@@ -338,7 +319,7 @@ __global__ void __launch_bounds__((XCfg<R, M>::NT))
   // X -> Z of the half-length complex inverse transform, per component
   for (int c = 0; c < 3; c++) {
     const Cx<R> *src = Hv + ((size_t)c * nrows + rowc) * NZD * D.NXp;
-    for (int k = t; k <= nx; k += T) wl[k] = src[hoff<R, 0>(NZD, D.NXp, zp, k)];
+    for (int k = t; k <= nx; k += T) wl[k] = src[hoff<R>(NZD, D.NXp, zp, k)];
     __syncthreads();
     Cx<R> *vl = vel + (c * G + g) * LS;
     for (int k = t; k < M; k += T) {
@@ -401,7 +382,7 @@ __global__ void __launch_bounds__((XCfg<R, M>::NT))
       const Cx<R> Z = wl[k], Zc = conj(wl[(M - k) % M]);
       const Cx<R> e = twh[k];  // exp(-i pi k / M)
       const Cx<R> s = (Z + Zc) * (R)0.5, d = (Z - Zc) * (R)0.5;
-      dst[hoff<R, 1>(NZD, D.NXp, zp, k)] = s - mul_i(e * d);
+      dst[hoff<R>(NZD, D.NXp, zp, k)] = s - mul_i(e * d);
     }
     __syncthreads();
   }
@@ -422,12 +403,33 @@ template <int M>
 struct XL {
   static constexpr int T = M / 8;
   static constexpr int E = M / T;
-  static constexpr int NTH = T < 64 ? 64 : T;  // threads per block
-  static constexpr int LPB = NTH / T;
+  // lines per block: at least four neighbouring z points (a tile's four
+  // points are one 128-byte run of the product buffer), a wavefront at least
+  static constexpr int LPB = 64 / T > 4 ? 64 / T : 4;
+  static constexpr int NTH = LPB * T;  // threads per block
   static constexpr int PL = M + M / 8;
+  // kept x modes a line can have: nx + 1 <= 2 nxd / 3 (the 3/2 rule)
+  static constexpr int NXS = 2 * M / 3 + 2;
 };
 
 __device__ __forceinline__ int padx(int p) { return p + (p >> 3); }
+
+// The exchange between two Stockham stages of a line: a line of up to 64
+// threads lives in one wavefront and in its own part of LDS, so the
+// wavefront need only wait for its own LDS traffic (the LDS serves a
+// wavefront's requests in order; the fences keep the compiler from moving
+// them across); the 1024-point lines span two wavefronts and take the
+// block's barrier.
+template <int M>
+__device__ __forceinline__ void xl_sync() {
+  if constexpr (XL<M>::T <= 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
 
 // The x pass's twiddles in LDS, stage by stage: a stage of stride S and
 // radix RAD (not the last) multiplies output k2 of butterfly n1 by
@@ -477,7 +479,7 @@ __device__ __forceinline__ void xl_fft(Cx<R> (&v)[XL<M>::E], Cx<R> *L, int t, co
         for (int n2 = 0; n2 < RAD; n2++) c[j][n2] = L[padx(q + S * (n1 + Ls * n2))];
       }
     }
-    __syncthreads();  // every read of L done before it is written again
+    xl_sync<M>();  // every read of L done before it is written again
 #pragma unroll
     for (int j = 0; j < NB; j++) {
       dft<R, SIGN, RAD>(c[j]);
@@ -504,7 +506,7 @@ __device__ __forceinline__ void xl_fft(Cx<R> (&v)[XL<M>::E], Cx<R> *L, int t, co
 #pragma unroll
         for (int k2 = 0; k2 < RAD; k2++) L[padx(q + S * k2 + S * RAD * n1)] = c[j][k2];
       }
-      __syncthreads();
+      xl_sync<M>();
       xl_fft<R, M, SIGN, ST + 1, S * RAD>(v, L, t, tw);
     }
   }
@@ -520,9 +522,9 @@ __device__ __forceinline__ void xl_c2r_load(Cx<R> (&dst)[XL<M>::E], const Cx<R> 
   for (int e = 0; e < E; e++) {
     const int k = t + T * e;
     Cx<R> a{(R)0, (R)0}, b{(R)0, (R)0};
-    if (ok && k <= nx) a = src[hoff<R, 0>(NZD, NXp, zp, k)];
+    if (ok && k <= nx) a = src[hoff<R>(NZD, NXp, zp, k)];
     if (k == 0) a.y = (R)0;  // c2r: the imaginary part of the kx = 0 bin is ignored
-    if (ok && k > 0 && M - k <= nx) b = conj(src[hoff<R, 0>(NZD, NXp, zp, M - k)]);
+    if (ok && k > 0 && M - k <= nx) b = conj(src[hoff<R>(NZD, NXp, zp, M - k)]);
     const Cx<R> ep = conj(twh[k]);  // exp(+i pi k / M)
     dst[e] = (a + b) + mul_i(ep * (a - b));
   }
@@ -543,18 +545,24 @@ __global__ void __launch_bounds__(XL<M>::NTH, (xline_min_waves<R, M>()))
     k_xline(const Cx<R> *__restrict__ Hv, Cx<R> *__restrict__ Hp, Dims D, int nrows, const Cx<R> *__restrict__ twx,
             const Cx<R> *__restrict__ twh, const double *__restrict__ dyl, double *__restrict__ cfl_part,
             int want_cfl, int row0) {
-  constexpr int T = XL<M>::T, E = XL<M>::E, LPB = XL<M>::LPB, PL = XL<M>::PL;
+  constexpr int T = XL<M>::T, E = XL<M>::E, LPB = XL<M>::LPB, PL = XL<M>::PL, NXS = XL<M>::NXS;
+  constexpr int NTH = XL<M>::NTH, HT = htile<R>();
   __shared__ Cx<R> lds[LPB * PL];
+  __shared__ Cx<R> stg[LPB * NXS];  // the block's kept modes of one product, [line][ix]
   __shared__ Cx<R> twl[xtw_size<M>() > 0 ? xtw_size<M>() : 1];  // the staged twiddles
   for (int k = threadIdx.x; k < xtw_size<M>(); k += blockDim.x) twl[k] = twx[xtw_exponent<M>(k)];
   __syncthreads();
   const int g = threadIdx.x / T, t = threadIdx.x % T;
   const int NZD = D.nzd, nx = D.nx;
-  // tiled Hv (FP32): neighbouring z points share cache lines, so neighbouring
+  // tiled Hv: neighbouring z points share cache lines, so neighbouring
   // blocks stay on one XCD (one L2)
-  const int line = (htile<R>() > 0 ? xcd_linear() : (int)blockIdx.x) * LPB + g;
+  const int line0 = xcd_linear() * LPB, line = line0 + g;
   const bool ok = line < NZD * nrows;
   const int rowc = ok ? line / NZD : 0, zp = ok ? line - rowc * NZD : 0;
+  // where the block's lines start in a product plane of the tiled buffer
+  // (-1: past the last line), for the stores of every product
+  __shared__ long long lbase[LPB];
+  if (t == 0) lbase[g] = ok ? (long long)rowc * NZD * D.NXp + (long long)zp * HT : -1;
   Cx<R> *L = lds + g * PL;
   Cx<R> u[E], v[E], w[E];
   // C2R pre-processing of u, v, w straight from the x modes in HBM
@@ -616,17 +624,30 @@ __global__ void __launch_bounds__(XL<M>::NTH, (xline_min_waves<R, M>()))
     // R2C post-processing needs Z(k) and Z(M - k)
 #pragma unroll
     for (int e = 0; e < E; e++) L[padx(t + T * e)] = z[e];
+    xl_sync<M>();
+    // every line of the block done with the previous product's tiles
+    // before stg is written again
     __syncthreads();
-    Cx<R> *dst = Hp + ((size_t)p * nrows + rowc) * plane;
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int k = t + T * e;
-      if (ok && k <= nx) {
+      if (k <= nx) {
         const Cx<R> Z = z[e], Zc = conj(L[padx((M - k) & (M - 1))]);
         const Cx<R> ep = twh[k];  // exp(-i pi k / M)
         const Cx<R> sm = (Z + Zc) * (R)0.5, df = (Z - Zc) * (R)0.5;
-        dst[hoff<R, 1>(NZD, D.NXp, zp, k)] = sm - mul_i(ep * df);
+        stg[g * NXS + k] = sm - mul_i(ep * df);
       }
+    }
+    __syncthreads();
+    // the block's lines to the tiles: consecutive threads take the lines of
+    // one tile's HT modes, so each tile's run of LPB z points is one store
+    const int nt = (nx + HT) / HT;
+    Cx<R> *const hp = Hp + (size_t)p * nrows * plane;
+    for (int i = threadIdx.x; i < nt * LPB * HT; i += NTH) {
+      const int tile = i / (LPB * HT), r = i - tile * (LPB * HT), gl = r / HT;
+      const int ix = tile * HT + (r - gl * HT);
+      const long long b = lbase[gl];
+      if (ix <= nx && b >= 0) hp[b + (long long)tile * NZD * HT + (r - gl * HT)] = stg[gl * NXS + ix];
     }
   }
 }
@@ -664,6 +685,7 @@ static int run_xpass(const Dims &D, int mode, const void *Hv, void *Hp, int row0
                      const double *dyl, double *cfl_part, int want_cfl, int *n_part, hipStream_t s) {
   if constexpr ((M & (M - 1)) == 0 && M >= 64) {
     typedef XL<M> C;
+    if (D.nx + 1 > C::NXS) return hipErrorInvalidValue;
     const int lines = D.nzd * nrows;
     dim3 grid((lines + C::LPB - 1) / C::LPB);
     if (n_part) *n_part = grid.x;

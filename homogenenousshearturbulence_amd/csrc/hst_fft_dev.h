@@ -10,6 +10,26 @@
 
 namespace hst {
 
+// ------------------------------------------------------------------ transform buffers
+
+// Layout of one (component, row) plane of the transform buffers Hv (the
+// velocities after the z inverse transform) and Hp (the products after the x
+// forward transform): nzd z points x NXp x modes, tiled by HT modes (4 in
+// FP32, 2 in FP64), each tile z-major ([ix / HT][zp][ix % HT]).  A block of
+// a z transform, which takes HT modes, then reads (k_zfft) or writes (k_zift)
+// one contiguous range; a block of the x pass, which takes four neighbouring
+// z points, writes each tile's four points as one 128-byte run (its lines
+// staged in LDS) and reads its lines in HT-mode pieces that the block's other
+// lines share.
+template <typename R>
+__host__ __device__ constexpr int htile() { return sizeof(R) == 4 ? 4 : 2; }
+template <typename R>
+__device__ __forceinline__ size_t hoff(int NZD, int NXp, int zp, int ix) {
+  constexpr int HT = htile<R>();
+  (void)NXp;
+  return (size_t)(ix / HT) * ((size_t)NZD * HT) + (size_t)zp * HT + ix % HT;
+}
+
 // ------------------------------------------------------------------ radix plan
 
 __host__ __device__ constexpr int pow2_part(int N) { return N % 3 == 0 ? N / 3 : N; }
