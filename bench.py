@@ -224,7 +224,6 @@ def main():
     per_launch = dom["bytes"] / dom["launches"]
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     # HBM bytes per launch of that kernel from the FETCH_SIZE / WRITE_SIZE runs
-    # (profiles/traffic_box256_y513_shift_mixed.md), scaled by the copy calibration
     # (profiles/traffic_*.md), scaled by the copy calibration; null when the
     # kernels changed since (the sources' hash differs)
     traffic = None

@@ -53,6 +53,11 @@ class Plan:
     batch: int = 65536
     lr: float = 1e-3
     seed: int = 1
+    # weights of the six stresses (uu vv ww uv vw uw) in the loss: a target's
+    # standard deviation in the standardisation is divided by sqrt(weight), so
+    # its standardised errors count weight times more (the network and its
+    # de-standardised output are the same model otherwise)
+    target_weight: tuple = (1.0, 1.0, 1.0, 1.0, 1.0, 1.0)
 
     @property
     def record_times(self):
@@ -111,7 +116,10 @@ def train(plan: Plan, c: closure.Closure, log=print):
     """Step 2."""
     npts = plan.les.ny * plan.les.nzd * 2 * plan.les.nxd
     n_train = npts * (len(plan.t_train) - plan.holdout)
-    c.normalise()
+    nm = c.normalise()
+    if any(w != 1.0 for w in plan.target_weight):
+        nm[24:30] = nm[24:30] / np.sqrt(np.asarray(plan.target_weight, dtype=np.float64))
+        c.set_norm(nm)
     c.init_params(plan.seed)
     t0 = _time.perf_counter()
     loss, val = c.train(plan.epochs, plan.batch, plan.lr, seed=plan.seed, n_train=n_train)

@@ -4,10 +4,15 @@ The DNS of the 128 x 257 x 128 grid (Re 1000) generates the data; the network is
 trained on the GPU by the library's kernels; two coarse runs on the 32 x 65 x 32
 grid start from the filtered DNS field, one with the closure attached and one
 without it (the plain method); the filtered DNS is the yardstick.  The measured
-errors (mean relative error of the resolved energy and dissipation over the
-records, and the mean |log| ratio of the x and z spectra over the upper half of
-the resolved wavenumbers at the last record) must all be smaller with the
-closure.
+errors (mean relative error of the resolved energy, dissipation and uv stress over
+the records, and the mean |log| ratio of the x and z spectra over the upper half
+of the resolved wavenumbers at the last record) are checked two ways: against
+stated bounds (so the closure's run is judged even where the plain run could not
+finish) and against the plain run.  The closure cuts the energy, dissipation and
+spectrum errors; it does not improve the uv stress, which it may make worse by at
+most half (a measured regression: 0.074 -> 0.078 on this plan, 0.072 -> 0.117 on
+the full one; weighting the shear stresses in the loss or training longer did not
+remove it).
 """
 import numpy as np
 import pytest
@@ -16,6 +21,10 @@ from homogenenousshearturbulence_amd import les
 
 pytestmark = pytest.mark.gpu
 
+# the closure run's errors against the filtered DNS (measured on this plan:
+# energy 0.49, dissipation 1.28, uv 0.078, spectra 1.45)
+BOUND = {"energy_error": 0.75, "dissipation_error": 2.0, "uv_error": 0.12, "spectrum_error": 2.0}
+
 
 def test_closure_gain_over_the_plain_coarse_run():
     plan = les.default_plan(quick=True)
@@ -23,11 +32,13 @@ def test_closure_gain_over_the_plain_coarse_run():
     plain, model = res["runs"]["plain"], res["runs"]["closure"]
     assert model["diverged_at"] is None
     assert np.all(np.isfinite(res["train_loss"])) and res["heldout_loss"][-1] < res["heldout_loss"][0]
-    if plain["diverged_at"] is not None:
-        return  # the plain run could not finish: the closure's gain is that it did
-    for k in ("energy_error", "dissipation_error", "spectrum_error"):
-        print(k, "plain", plain[k], "closure", model[k])
-        assert model[k] < 0.75 * plain[k], k
-    # the filtered DNS loses energy faster than the plain coarse run: the closure drains it
-    e_ref, e_plain, e_model = res["filtered_dns"]["energy"][-1], plain["energy"][-1], model["energy"][-1]
-    assert abs(e_model - e_ref) < abs(e_plain - e_ref)
+    for k, b in BOUND.items():
+        print(k, "plain", plain.get(k), "closure", model[k], "bound", b)
+        assert model[k] <= b, k
+    if plain["diverged_at"] is None:
+        for k in ("energy_error", "dissipation_error", "spectrum_error"):
+            assert model[k] < 0.75 * plain[k], k
+        assert model["uv_error"] <= 1.5 * plain["uv_error"]
+        # the filtered DNS loses energy faster than the plain coarse run: the closure drains it
+        e_ref, e_plain, e_model = res["filtered_dns"]["energy"][-1], plain["energy"][-1], model["energy"][-1]
+        assert abs(e_model - e_ref) < abs(e_plain - e_ref)
