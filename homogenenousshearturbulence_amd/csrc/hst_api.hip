@@ -97,6 +97,10 @@ struct State {
   int64_t bytes_held = 0;
   bool prof = false;
   bool serial = false;  // the oldrhs shift on the main stream (HST_SERIAL=1)
+  // the time step's dv/dy in one sweep (k_dy_window) when the D0 factor decays
+  // fast enough on this grid; HST_NO_DYWIN=1: the stored sweep of k_d0solve
+  bool dyw = false;
+  double *d_sig = nullptr;
   // a time step leaves eta in V1 and dv/dy in V3 (every row) instead of u
   // and w: the next step's transforms and right-hand sides read that form,
   // and every other call restores u, w first (restore_uw)
@@ -547,15 +551,75 @@ static const void *const *penta_tables(double lambda) {
   return S.ptab[k];
 }
 
+// The one-sweep dv/dy (k_dy_window) leaves out, for a row emitted from its
+// window, the eliminated rows more than dy_window_rows() above it, and for
+// the rows it does not correct, the start (b1, b2) and border terms.  All
+// three are set by the D0 factorisation alone, the same for every column:
+// this computes the border terms' share s1, s2 of the first rows (the
+// back substitution of -B1, -B2) and takes the kernel only where each left-out
+// share is below 1e-16 of the values it multiplies.
+static bool dy_window_setup(std::vector<double> &sig) {
+  const int mi = S.D.ny - 2, W = dy_window_rows();
+  if (mi < 2 * (W + 8) + W) return false;
+  const std::vector<double> &f = S.f0.rows;
+  auto U1 = [&](int i) { return f[(size_t)i * D0Factor::NCOL + D0Factor::U1]; };
+  auto U2 = [&](int i) { return f[(size_t)i * D0Factor::NCOL + D0Factor::U2]; };
+  // the back substitution of sg * add (or of zero) from the start (s1, s2)
+  auto back = [&](std::vector<double> &x, double s1, double s2, const std::vector<double> *add, double sg) {
+    x.assign((size_t)mi + 2, 0.0);
+    x[mi] = s1;
+    x[mi + 1] = s2;
+    for (int i = mi - 1; i >= 0; i--)
+      x[i] = (add ? sg * (*add)[i] : 0.0) - U1(i) * x[i + 1] - U2(i) * x[i + 2];
+  };
+  std::vector<double> b1(mi), b2(mi);
+  for (int i = 0; i < mi; i++) {
+    b1[i] = f[(size_t)i * D0Factor::NCOL + D0Factor::B1];
+    b2[i] = f[(size_t)i * D0Factor::NCOL + D0Factor::B2];
+  }
+  std::vector<double> s1, s2, t1, t2;
+  back(s1, 0.0, 0.0, &b1, -1.0);
+  back(s2, 0.0, 0.0, &b2, -1.0);
+  back(t1, 1.0, 0.0, nullptr, 0.0);
+  back(t2, 0.0, 1.0, nullptr, 0.0);
+  double worst = 0.0;
+  // border terms beyond the corrected rows; start terms below the exact tail
+  for (int i = W; i < mi; i++) worst = std::max(worst, std::fabs(s1[i]) + std::fabs(s2[i]));
+  for (int i = 0; i < mi - W - 8; i++) worst = std::max(worst, std::fabs(t1[i]) + std::fabs(t2[i]));
+  // an eliminated row's share of the rows more than W below it
+  for (int j = 0; j < mi; j++) {
+    double h1 = 1.0, h2 = 0.0, tail = 0.0;  // h at rows i + 1, i + 2 of the impulse at row j
+    for (int i = j - 1, d = 1; i >= 0 && d <= W + 64; i--, d++) {
+      const double h = -U1(i) * h1 - U2(i) * h2;
+      if (d > W) tail += std::fabs(h);
+      h2 = h1;
+      h1 = h;
+    }
+    worst = std::max(worst, tail);
+  }
+  if (!(worst < 1e-16)) return false;
+  sig.resize(2 * (size_t)W);
+  for (int i = 0; i < W; i++) {
+    sig[2 * i] = s1[i];
+    sig[2 * i + 1] = s2[i];
+  }
+  return true;
+}
+
 // after the solves: dv/dy, and u, w from it and eta (compact: dv/dy into V3,
 // eta left in V1), then the ghost rows of both at the shifts (sx, sz)
 static int recover(bool compact, double sx, double sz) {
   int rc;
   if (compact) {
-    rc = run("dvdy", 4.0 * bytes_field(), [&] {
-      return launch_d0solve(S.D, 1, 0, S.F[HST_V2], S.F[HST_V3], nullptr, S.XR[0], S.d_f0, S.fc, S.d_der, S.d_dyl,
-                            sx, sz, nullptr, nullptr, S.stream);
-    });
+    if (S.dyw)
+      rc = run("dvdy", 2.0 * bytes_field(), [&] {
+        return launch_dy_window(S.D, S.F[HST_V2], S.F[HST_V3], S.d_f0, S.fc, S.d_der, S.d_sig, sx, sz, S.stream);
+      });
+    else
+      rc = run("dvdy", 4.0 * bytes_field(), [&] {
+        return launch_d0solve(S.D, 1, 0, S.F[HST_V2], S.F[HST_V3], nullptr, S.XR[0], S.d_f0, S.fc, S.d_der,
+                              S.d_dyl, sx, sz, nullptr, nullptr, S.stream);
+      });
   } else {
     rc = run("dy_recover", 6.0 * bytes_field(), [&] {
       return launch_d0solve(S.D, 1, 1, S.F[HST_V2], S.F[HST_V1], S.F[HST_V3], S.XR[0], S.d_f0, S.fc, S.d_der,
@@ -655,7 +719,7 @@ static void release() {
   if (S.stream) (void)hipStreamSynchronize(S.stream);
   if (S.side) (void)hipStreamSynchronize(S.side);
   for (int i = 1; i <= 8; i++) if (S.F[i]) (void)hipFree(S.F[i]);
-  void *ptrs[] = {S.P, S.Hv, S.Hp, S.ws, S.d_der, S.d_y, S.d_dyl, S.d_f0, S.d_part, S.d_f0r};
+  void *ptrs[] = {S.P, S.Hv, S.Hp, S.ws, S.d_der, S.d_y, S.d_dyl, S.d_f0, S.d_part, S.d_f0r, S.d_sig};
   for (void *p : ptrs) if (p) (void)hipFree(p);
   for (auto &t : S.ptab)
     for (void *p : t) if (p) (void)hipFree(p);
@@ -821,6 +885,15 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
   HIPCHK(hipMemcpy(S.d_f0, S.f0.rows.data(), S.f0.rows.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&S.d_f0r, S.f0r.rows.size() * sizeof(double)));
   HIPCHK(hipMemcpy(S.d_f0r, S.f0r.rows.data(), S.f0r.rows.size() * sizeof(double), hipMemcpyHostToDevice));
+  {
+    std::vector<double> sig;
+    const char *e = std::getenv("HST_NO_DYWIN");
+    if (!(e && std::atoi(e) != 0) && dy_window_setup(sig)) {
+      HIPCHK(hipMalloc(&S.d_sig, sig.size() * sizeof(double)));
+      HIPCHK(hipMemcpy(S.d_sig, sig.data(), sig.size() * sizeof(double), hipMemcpyHostToDevice));
+      S.dyw = true;
+    }
+  }
   S.part_cap = std::max<size_t>((size_t)D.NYg * D.nzd, (size_t)((D.ncol + 255) / 256) * NSTAT + 64);
   HIPCHK(hipMalloc(&S.d_part, S.part_cap * sizeof(double)));
   HIPCHK(hipHostMalloc((void **)&S.h_part, S.part_cap * sizeof(double), hipHostMallocDefault));

@@ -119,6 +119,13 @@ int launch_d0solve(const Dims &D, int rhs_mode, int out_mode, const double2 *src
                    double2 *XR, const double *f0rows, const D0Const &fc, const double *der, const double *dyl,
                    double sx, double sz, double *part, int *n_part, hipStream_t s, const int *inlayer = nullptr,
                    int skip00 = 0);
+// D0 x = D1 src in one sweep (OUT 0 of launch_d0solve, the time step's dv/dy)
+// with the back substitution windowed over dy_window_rows() rows; sig holds
+// the back substitution of -B1, -B2 over the first dy_window_rows() rows
+// ([i][2]).  The caller checks that the factorisation decays fast enough.
+int launch_dy_window(const Dims &D, const double2 *src, double2 *dst, const double *f0rows, const D0Const &fc,
+                     const double *der, const double *sig, double sx, double sz, hipStream_t s);
+int dy_window_rows();
 // batched FP64 line transforms of the per-field entry points (hst_lines.hip)
 int launch_cfft_lines(int n, int sign, double2 *data, int64_t nlines, const void *tw, hipStream_t s);
 int launch_rfft_lines(int m, int dir, double2 *data, int64_t nlines, const void *tw, const void *twh,

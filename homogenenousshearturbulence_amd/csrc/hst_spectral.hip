@@ -881,6 +881,145 @@ int launch_d0solve(const Dims &D, int rhs_mode, int out_mode, const double2 *src
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ dv/dy in one sweep
+
+// D0 x = D1 v (the time step's dv/dy) in one pass over the rows, without the
+// workspace: the forward elimination is d0_forward's, but the eliminated rows
+// stay in registers, DYW_N = DYW_B + DYW_W of them, and every DYW_B rows the
+// back substitution runs from the top of that window down through its lowest
+// DYW_B rows, started from zero and without the border terms.  The inverse
+// of D0's unit upper factor decays by about 0.29 a row on these grids (the
+// host checks the bound for the deck, dy_window_ok), so a row emitted that
+// way misses contributions below 1e-16 of the column's values.  The rows
+// within DYW_W of the top stay in the window until the border is solved and
+// take the exact back substitution; the first DYW_W rows, where the border
+// columns have not decayed, are emitted without them and corrected at the end
+// by yb1 s1(i) + yb2 s2(i), s1, s2 the back substitution of -B1 and -B2 (the
+// same for every column; sig = [i][2]).  Per point v is read once and dv/dy
+// written once: 32 B against the 64 B of k_d0solve's stored sweep.
+constexpr int DYW_B = 8, DYW_W = 32, DYW_N = DYW_B + DYW_W;
+
+__global__ void __launch_bounds__(256, 2)
+    k_dy_window(Dims D, D0Const fc, const double *__restrict__ f0, const double *__restrict__ der,
+                const double2 *__restrict__ src, double2 *__restrict__ dst, const double *__restrict__ sig, double sx,
+                double sz) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= D.ncol) return;
+  int ix, iz;
+  col_ix_iz(D, col, ix, iz);
+  const size_t nc = D.ncol;
+  const int m = D.ny, mi = m - 2;
+  C w[5];  // the D1 stencil's window src(i - 2 .. i + 2)
+#pragma unroll
+  for (int k = 0; k < 4; k++) w[k + 1] = ld(src + (size_t)k * nc + col);
+  C x1{0, 0}, x2{0, 0}, sx0{0, 0}, sx1{0, 0};
+  // forward elimination of row i from src row i + 2 (array row i + 4)
+  auto fwd = [&](int i, const C &v) {
+    w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4]; w[4] = v;
+    C bx{0, 0};
+#pragma unroll
+    for (int j = -2; j <= 2; j++) bx = bx + DER(der, i, 1, j) * w[j + 2];
+    if (i >= 2) bx = bx - F0(i, 0) * x2;
+    if (i >= 1) bx = bx - F0(i, 1) * x1;
+    bx = bx * F0(i, 2);
+    sx0 = sx0 + F0(i, 7) * bx;
+    sx1 = sx1 + F0(i, 8) * bx;
+    x2 = x1;
+    x1 = bx;
+    return bx;
+  };
+  C ring[DYW_N];  // ring[k]: the eliminated row r + k
+  {
+    C v[DYW_N];
+#pragma unroll
+    for (int k = 0; k < DYW_N; k++) v[k] = ld(src + (size_t)(k + 4) * nc + col);
+#pragma unroll
+    for (int k = 0; k < DYW_N; k++) ring[k] = fwd(k, v[k]);
+  }
+  // the window's lowest DYW_B rows, substituted from its top with a zero start
+  auto emit_low = [&](int r) {
+    C h1{0, 0}, h2{0, 0};
+#pragma unroll
+    for (int k = DYW_N - 1; k >= 0; k--) {
+      const int i = r + k;
+      const C h = ring[k] - F0(i, 3) * h1 - F0(i, 4) * h2;
+      if (k < DYW_B) st(dst + (size_t)(i + 2) * nc + col, h);
+      h2 = h1;
+      h1 = h;
+    }
+  };
+  int r = 0;
+  for (; r + DYW_N + DYW_B <= mi; r += DYW_B) {
+    C v[DYW_B];
+#pragma unroll
+    for (int k = 0; k < DYW_B; k++) v[k] = ld(src + (size_t)(r + DYW_N + k + 4) * nc + col);
+    emit_low(r);
+#pragma unroll
+    for (int k = 0; k < DYW_W; k++) ring[k] = ring[k + DYW_B];
+#pragma unroll
+    for (int k = 0; k < DYW_B; k++) ring[DYW_W + k] = fwd(r + DYW_N + k, v[k]);
+  }
+  // the last, partial block: rem < DYW_B rows are left to eliminate
+  const int rem = mi - (r + DYW_N);
+  {
+    C v[DYW_B];
+#pragma unroll
+    for (int k = 0; k < DYW_B; k++)
+      if (k < rem) v[k] = ld(src + (size_t)(r + DYW_N + k + 4) * nc + col);
+    emit_low(r);
+#pragma unroll
+    for (int k = 0; k < DYW_W; k++) ring[k] = ring[k + DYW_B];
+#pragma unroll
+    for (int k = 0; k < DYW_B; k++)
+      if (k < rem) ring[DYW_W + k] = fwd(r + DYW_N + k, v[k]);
+    r += DYW_B;
+  }
+  // the border rows (src rows m, m + 1 for the stencils at m - 2, m - 1)
+  w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4]; w[4] = ld(src + (size_t)(m + 2) * nc + col);
+  C r25{0, 0};
+#pragma unroll
+  for (int j = -2; j <= 2; j++) r25 = r25 + DER(der, m - 2, 1, j) * w[j + 2];
+  w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4]; w[4] = ld(src + (size_t)(m + 3) * nc + col);
+  C r27{0, 0};
+#pragma unroll
+  for (int j = -2; j <= 2; j++) r27 = r27 + DER(der, m - 1, 1, j) * w[j + 2];
+  const C ph = cexpi(-(D.alfa0 * ix * sx + D.beta0 * iz * sz));
+  C b1, b2;
+  d0_border(fc, ph, x1, x2, sx0, sx1, r25, r27, b1, b2);
+  const C cph = conj(ph);
+  const C yb1 = cph * b1, yb2 = cph * b2;
+  st(dst + (size_t)(m - 2 + 2) * nc + col, b1);
+  st(dst + (size_t)(m - 1 + 2) * nc + col, b2);
+  // the rows r .. mi - 1 still in the window: the exact back substitution
+  const int top = mi - r;  // DYW_W + rem of them
+  C xk1 = b1, xk2 = b2;
+#pragma unroll
+  for (int k = DYW_N - 1; k >= 0; k--) {
+    if (k < top) {
+      const int i = r + k;
+      const C xk = ring[k] - F0(i, 5) * yb1 - F0(i, 6) * yb2 - F0(i, 3) * xk1 - F0(i, 4) * xk2;
+      st(dst + (size_t)(i + 2) * nc + col, xk);
+      xk2 = xk1;
+      xk1 = xk;
+    }
+  }
+  // the border columns' share of the first rows
+  for (int i = 0; i < DYW_W; i++) {
+    const size_t o = (size_t)(i + 2) * nc + col;
+    st(dst + o, ld(dst + o) + sig[2 * i] * yb1 + sig[2 * i + 1] * yb2);
+  }
+}
+
+int launch_dy_window(const Dims &D, const double2 *src, double2 *dst, const double *f0rows, const D0Const &fc,
+                     const double *der, const double *sig, double sx, double sz, hipStream_t s) {
+  if (D.ny - 2 < 2 * DYW_N + DYW_W) return hipErrorInvalidValue;
+  dim3 grid((D.ncol + 255) / 256);
+  hipLaunchKernelGGL(k_dy_window, grid, dim3(256), 0, s, D, fc, f0rows, der, src, dst, sig, sx, sz);
+  return hipGetLastError();
+}
+
+int dy_window_rows() { return DYW_W; }
+
 // ------------------------------------------------------------------ line solves of a column range
 
 // line_solve over the columns col0 .. col0 + ncols - 1 only, on arrays that

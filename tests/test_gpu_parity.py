@@ -538,3 +538,45 @@ def test_exact_shift_without_factor_tables(O, precision, tol):
         assert refops.rel_l2(g.V, Vn) <= tol
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("exact,ystretch", [(True, 0.0), (False, 0.0), (True, 2.0)])
+def test_dvdy_one_sweep(O, exact, ystretch):
+    """The time step's dv/dy in one sweep (taken when ny - 2 >= 112 and the D0
+    factorisation decays fast enough): three FP64 steps of a thin deck with
+    ny = 257 against the oracle and against the stored sweep (HST_NO_DYWIN=1),
+    uniform and stretched grid; the profile shows which kernel ran (the
+    one-sweep kernel moves half the bytes)."""
+    d = dataclasses.replace(read_input(deck_path("small")), nx=6, nz=6, ny=257, kpeak=3.0, alfa0=1.0, beta0=1.0,
+                            ly=2 * np.pi, re=1000.0, amplitude=0.002, deltat=0.001, exact_shift=exact,
+                            ystretch=ystretch)
+    o = O.run_deck(d, nsteps=3)
+
+    def dvdy_bytes(g):
+        g.profile(True)
+        g.timestep(d.deltat)
+        g.sync()
+        row = [r for r in g.profile_table() if r["kernel"] == "dvdy"]
+        g.profile(False)
+        return row[0]["bytes"] / row[0]["launches"]
+
+    g = hst.run_deck(d, hst.FP64, nsteps=3)
+    try:
+        assert g.time == o.time
+        V1 = g.V
+        assert rel(V1, o.V) <= 1e-11
+        one = dvdy_bytes(g)
+    finally:
+        g.close()
+    os.environ["HST_NO_DYWIN"] = "1"
+    try:
+        g = hst.run_deck(d, hst.FP64, nsteps=3)
+    finally:
+        del os.environ["HST_NO_DYWIN"]
+    try:
+        assert rel(g.V, V1) <= 1e-13
+        two = dvdy_bytes(g)
+    finally:
+        g.close()
+    field = d.ny * (d.nx + 1) * (2 * d.nz + 1) * 16.0
+    assert one == pytest.approx(2 * field) and two == pytest.approx(4 * field)
