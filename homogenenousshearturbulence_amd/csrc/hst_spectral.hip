@@ -1290,13 +1290,13 @@ int launch_exact_shift(const Dims &D, int nq, void *const *q, const int *f32, do
 // elimination takes them.  Rows 0, 1 of x come from the reversed border
 // system, rows ny-2, ny-1 first from the record (for the ghost rows below row
 // 0), then from the back substitution.
-// Loads two rows ahead in each sweep.  The launch bound asks for two
-// wavefronts per SIMD; the kernel still fits in 155 VGPRs (three per SIMD),
-// and the compiler's schedule under this bound measured 4 % faster than under
-// a bound of three at nx = nz = 256, ny = 513.  Deeper prefetch ran slower.
+// Loads two rows ahead in each sweep, the phase factors of the second sweep
+// with them (multiplied only where used: 5 % faster than carrying their
+// product, which made each prefetch wait for its loads).  Three wavefronts
+// per SIMD (168 VGPRs).  Deeper prefetch ran slower.
 // One system alone (half the launch) takes 0.48 of the time of both: the
 // kernel is bound by HBM bandwidth, not by the second, partly filled round.
-constexpr int SP_WAVES = 2, SP_PF = 2;
+constexpr int SP_WAVES = 3, SP_PF = 2;
 template <int K>
 __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, int ix, int iz, int g,
                                                  const D0Const &fcr,
@@ -1376,18 +1376,26 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, int ix,
     W0 = W1; W1 = W2; W2 = W3; W3 = xnext;
     return acc;
   };
+  // the row's phase factors travel with it unmultiplied: a product formed at
+  // load time would wait for them there, and the loads issued ahead would be
+  // waited for one row after another
   struct Row {
-    C xr, ph;
+    C xr, px, pz;
   };
   C xk1 = b1, xk2 = b2;
   rows_descending<SP_PF, Row>(
-      mi, [&](int k) { return Row{ld(XRs + (size_t)k * nc + col), phase(m - 1 - k)}; },
+      mi,
+      [&](int k) {
+        const int i = m - 1 - k;
+        return Row{ld(XRs + (size_t)k * nc + col), ld(PX + (size_t)i * D.NXm + ix),
+                   ld(PZ + (size_t)i * D.NZm + az + D.nz)};
+      },
       [&](int k, const Row &r) {
         const C xk = r.xr - F0(k, 5) * yb1 - F0(k, 6) * yb2 - F0(k, 3) * xk1 - F0(k, 4) * xk2;
         xk2 = xk1;
         xk1 = xk;
         const int i = m - 1 - k - 2;  // x'(m-1-k) completes row m-3-k < mi
-        feed(i, reweight(i, xk * r.ph));
+        feed(i, reweight(i, xk * (r.px * (iz < 0 ? conj(r.pz) : r.pz))));
       });
   // the border rows' right-hand sides (the loop above covers the rows < mi,
   // so that its body has no branch on the row)
