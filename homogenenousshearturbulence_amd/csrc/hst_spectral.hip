@@ -552,6 +552,9 @@ __device__ __forceinline__ void penta_back(const Dims &D, int g, double lam, dou
                                            const double *__restrict__ der, const double4 *__restrict__ tab,
                                            C zb1, C zb2, C x1, C x2, LOADX loadx, EMIT emit) {
   const int mi = D.ny - 2, ng = (D.nx + 1) * (D.nz + 1);
+  // the next chunk's checkpoint is loaded while this chunk is substituted
+  // (3-4 % faster in the fused shift and solve than loading it at its chunk)
+  double4 ta = tab[(size_t)(2 * ((mi - 1) / PC)) * ng + g], tb = tab[(size_t)(2 * ((mi - 1) / PC) + 1) * ng + g];
   for (int c = (mi - 1) / PC; c >= 0; c--) {
     const int c0 = c * PC, n = min(PC, mi - c0);
     C xr[PC];
@@ -559,7 +562,7 @@ __device__ __forceinline__ void penta_back(const Dims &D, int g, double lam, dou
     for (int j = 0; j < PC; j++)
       if (j < n) xr[j] = loadx(c0 + j);
     PentaFactor f;
-    f.restore(tab[(size_t)(2 * c) * ng + g], tab[(size_t)(2 * c + 1) * ng + g]);
+    f.restore(ta, tb);
     double t1[PC], t2[PC], pp[PC], qq[PC];
 #pragma unroll
     for (int j = 0; j < PC; j++) {
@@ -567,6 +570,10 @@ __device__ __forceinline__ void penta_back(const Dims &D, int g, double lam, dou
         double l2, l1, rp;
         f.step<K>(c0 + j, lam, D.ni, kk, der, l2, l1, rp, t1[j], t2[j], pp[j], qq[j]);
       }
+    }
+    if (c > 0) {
+      ta = tab[(size_t)(2 * c - 2) * ng + g];
+      tb = tab[(size_t)(2 * c - 1) * ng + g];
     }
 #pragma unroll
     for (int j = PC - 1; j >= 0; j--) {
