@@ -134,6 +134,30 @@ __device__ __forceinline__ void xcd_tile(int &tile, int &row, int &comp) {
   comp = l / (nt * nr);
 }
 
+// dst[idx(k)] = f(k) for k < N by the NT threads of a block (blockDim.x ==
+// NT): every load of a thread is issued before its first store.  A plain
+// strided loop stores each value as soon as it is loaded, so it waits for
+// one load after another.
+template <int N, int NT, typename T, class F, class IDX>
+__device__ __forceinline__ void lds_fill(T *dst, F f, IDX idx) {
+  constexpr int PER = (N + NT - 1) / NT;
+  T v[PER];
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int k = (int)threadIdx.x + u * NT;
+    if (N % NT == 0 || k < N) v[u] = f(k);
+  }
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int k = (int)threadIdx.x + u * NT;
+    if (N % NT == 0 || k < N) dst[idx(k)] = v[u];
+  }
+}
+template <int N, int NT, typename T, class F>
+__device__ __forceinline__ void lds_fill(T *dst, F f) {
+  lds_fill<N, NT>(dst, f, [](int k) { return k; });
+}
+
 // ------------------------------------------------------------------ k_zift
 
 // z inverse transform of the three velocity components for the rows of a
@@ -159,25 +183,25 @@ __global__ void __launch_bounds__((ZCfg<R, NZD, false>::NT))
   }
   __shared__ Cx<R> lds[G * LS];
   __shared__ Cx<R> twl[ztw_lds<R>() ? NZD : 1];  // the twiddles: lanes of a stage read scattered entries
-  if constexpr (ztw_lds<R>())
-    for (int k = threadIdx.x; k < NZD; k += blockDim.x) twl[k] = tw[k];
   const Cx<R> *twp = ztw_lds<R>() ? twl : tw;
   const int row = row0 + rowc, nz = D.nz;
   const size_t rb = (size_t)row * D.ncol;
   if (!pair) {
     const int cc = CMP ? 1 : c, ix0 = tile * G;
     const double2 *src = (cc == 0 ? V1 : (cc == 1 ? V2 : V3)) + rb + (size_t)ix0 * D.NZm;
-    for (int j = 0; j < G; j++) {
-      const bool ok = ix0 + j < D.NXm;
-      for (int k = threadIdx.x; k < NZD; k += blockDim.x) {
-        Cx<R> v{(R)0, (R)0};
-        if (ok) {
-          if (k <= nz) v = cx_from<R>(src[(size_t)j * D.NZm + k + nz]);
-          else if (k >= NZD - nz) v = cx_from<R>(src[(size_t)j * D.NZm + (k - NZD) + nz]);
-        }
-        lds[j * LS + k] = v;
-      }
-    }
+    if constexpr (ztw_lds<R>()) lds_fill<NZD, NT>(twl, [&](int k) { return tw[k]; });
+    lds_fill<G * NZD, NT>(
+        lds,
+        [&](int e) {
+          const int j = e / NZD, k = e - j * NZD;
+          Cx<R> v{(R)0, (R)0};
+          if (ix0 + j < D.NXm) {
+            if (k <= nz) v = cx_from<R>(src[(size_t)j * D.NZm + k + nz]);
+            else if (k >= NZD - nz) v = cx_from<R>(src[(size_t)j * D.NZm + (k - NZD) + nz]);
+          }
+          return v;
+        },
+        [&](int e) { const int j = e / NZD; return j * LS + (e - j * NZD); });
     __syncthreads();
     const int g = threadIdx.x / T, t = threadIdx.x % T;
     fft_line<R, NZD, T, +1>(lds + g * LS, t, twp);
@@ -204,6 +228,8 @@ __global__ void __launch_bounds__((ZCfg<R, NZD, false>::NT))
         }
       }
     }
+    // the twiddles, while the lines' loads are in flight
+    if constexpr (ztw_lds<R>()) lds_fill<NZD, NT>(twl, [&](int k) { return tw[k]; });
 #pragma unroll
     for (int j = 0; j < HG; j++) {
       const int ix = ix0 + j;
@@ -241,19 +267,38 @@ __global__ void __launch_bounds__((ZCfg<R, NZD, true>::NT))
     k_zfft(const Cx<R> *__restrict__ Hp, Cx<R> *__restrict__ P0, size_t pstride, Dims D, int row0, int nrows,
            const Cx<R> *__restrict__ tw) {
   constexpr int G = ZCfg<R, NZD, true>::G, T = ZCfg<R, NZD, true>::T, LS = ZCfg<R, NZD, true>::LS;
+  constexpr int NT = ZCfg<R, NZD, true>::NT;
   __shared__ Cx<R> lds[G * LS];
   __shared__ Cx<R> twl[ztw_lds<R>() ? NZD : 1];
-  if constexpr (ztw_lds<R>())
-    for (int k = threadIdx.x; k < NZD; k += blockDim.x) twl[k] = tw[k];
-  const Cx<R> *twp = ztw_lds<R>() ? twl : tw;
   int tile, rowc, p;
   xcd_tile(tile, rowc, p);
   const int row = row0 + rowc, ix0 = tile * G;
   const Cx<R> *src = Hp + ((size_t)p * nrows + rowc) * NZD * D.NXp;
-  for (int e = threadIdx.x; e < G * NZD; e += blockDim.x) {
-    const int j = e % G, zp = e / G;
-    lds[j * LS + zp] = (ix0 + j < D.NXm) ? src[hoff<R>(NZD, D.NXp, zp, ix0 + j)] : Cx<R>{(R)0, (R)0};
+  // the lines and the twiddles: every load in flight before the first store
+  constexpr int PER = (G * NZD + NT - 1) / NT, PT = ztw_lds<R>() ? (NZD + NT - 1) / NT : 0;
+  Cx<R> v[PER], tv[PT > 0 ? PT : 1];
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int e = threadIdx.x + u * NT, j = e % G, zp = e / G;
+    if ((G * NZD) % NT == 0 || e < G * NZD)
+      v[u] = (ix0 + j < D.NXm) ? src[hoff<R>(NZD, D.NXp, zp, ix0 + j)] : Cx<R>{(R)0, (R)0};
   }
+#pragma unroll
+  for (int u = 0; u < PT; u++) {
+    const int k = threadIdx.x + u * NT;
+    if (NZD % NT == 0 || k < NZD) tv[u] = tw[k];
+  }
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int e = threadIdx.x + u * NT;
+    if ((G * NZD) % NT == 0 || e < G * NZD) lds[(e % G) * LS + e / G] = v[u];
+  }
+#pragma unroll
+  for (int u = 0; u < PT; u++) {
+    const int k = threadIdx.x + u * NT;
+    if (NZD % NT == 0 || k < NZD) twl[k] = tv[u];
+  }
+  const Cx<R> *twp = ztw_lds<R>() ? twl : tw;
   __syncthreads();
   const int g = threadIdx.x / T, t = threadIdx.x % T;
   const int nz = D.nz;
