@@ -559,19 +559,30 @@ __device__ __forceinline__ void xl_fft(Cx<R> (&v)[XL<M>::E], Cx<R> *L, int t, co
 
 // Z(k) = (X(k) + conj X(M-k)) + i exp(i pi k/M) (X(k) - conj X(M-k)) at k = t + T e:
 // the half-length complex input of the length-2M inverse real transform.
+// Every load of the line is issued before the first is used: with the loads
+// and the arithmetic interleaved per element, each element's conditional
+// loads were waited for before the next element's went out.
 template <typename R, int M>
 __device__ __forceinline__ void xl_c2r_load(Cx<R> (&dst)[XL<M>::E], const Cx<R> *__restrict__ src, int NZD, int NXp,
                                             int zp, bool ok, int t, int nx, const Cx<R> *__restrict__ twh) {
   constexpr int T = XL<M>::T, E = XL<M>::E;
+  Cx<R> a[E], b[E], ep[E];
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int k = t + T * e;
-    Cx<R> a{(R)0, (R)0}, b{(R)0, (R)0};
-    if (ok && k <= nx) a = src[hoff<R>(NZD, NXp, zp, k)];
-    if (k == 0) a.y = (R)0;  // c2r: the imaginary part of the kx = 0 bin is ignored
-    if (ok && k > 0 && M - k <= nx) b = conj(src[hoff<R>(NZD, NXp, zp, M - k)]);
-    const Cx<R> ep = conj(twh[k]);  // exp(+i pi k / M)
-    dst[e] = (a + b) + mul_i(ep * (a - b));
+    a[e] = Cx<R>{(R)0, (R)0};
+    b[e] = Cx<R>{(R)0, (R)0};
+    if (ok && k <= nx) a[e] = src[hoff<R>(NZD, NXp, zp, k)];
+    if (ok && k > 0 && M - k <= nx) b[e] = src[hoff<R>(NZD, NXp, zp, M - k)];
+    ep[e] = twh[k];
+  }
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int k = t + T * e;
+    Cx<R> av = a[e];
+    if (k == 0) av.y = (R)0;  // c2r: the imaginary part of the kx = 0 bin is ignored
+    const Cx<R> bv = conj(b[e]);
+    dst[e] = (av + bv) + mul_i(conj(ep[e]) * (av - bv));  // conj(twh[k]) = exp(+i pi k / M)
   }
 }
 
@@ -594,7 +605,12 @@ __global__ void __launch_bounds__(XL<M>::NTH, (xline_min_waves<R, M>()))
   constexpr int NTH = XL<M>::NTH, HT = htile<R>();
   __shared__ Cx<R> lds[LPB * PL];
   __shared__ Cx<R> twl[xtw_size<M>() > 0 ? xtw_size<M>() : 1];  // the staged twiddles
-  for (int k = threadIdx.x; k < xtw_size<M>(); k += blockDim.x) twl[k] = twx[xtw_exponent<M>(k)];
+  lds_fill<xtw_size<M>(), NTH>(twl, [&](int k) { return twx[xtw_exponent<M>(k)]; });
+  // exp(-i pi k / M) of the R2C post-processing, read by every product: from
+  // LDS (read from global memory inside the product loop, each element waited
+  // for its own load)
+  __shared__ Cx<R> twhl[M];
+  lds_fill<M, NTH>(twhl, [&](int k) { return twh[k]; });
   __syncthreads();
   const int g = threadIdx.x / T, t = threadIdx.x % T;
   const int NZD = D.nzd, nx = D.nx;
@@ -678,7 +694,7 @@ __global__ void __launch_bounds__(XL<M>::NTH, (xline_min_waves<R, M>()))
       const int k = t + T * e;
       if (k <= nx) {
         const Cx<R> Z = z[e], Zc = conj(L[padx((M - k) & (M - 1))]);
-        const Cx<R> ep = twh[k];  // exp(-i pi k / M)
+        const Cx<R> ep = twhl[k];  // exp(-i pi k / M)
         z[e] = (Z + Zc) - mul_i(ep * (Z - Zc));
       }
     }
