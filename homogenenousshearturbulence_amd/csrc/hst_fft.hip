@@ -194,12 +194,10 @@ __global__ void __launch_bounds__((ZCfg<R, NZD, false>::NT))
         lds,
         [&](int e) {
           const int j = e / NZD, k = e - j * NZD;
-          Cx<R> v{(R)0, (R)0};
-          if (ix0 + j < D.NXm) {
-            if (k <= nz) v = cx_from<R>(src[(size_t)j * D.NZm + k + nz]);
-            else if (k >= NZD - nz) v = cx_from<R>(src[(size_t)j * D.NZm + (k - NZD) + nz]);
-          }
-          return v;
+          const bool in = ix0 + j < D.NXm && (k <= nz || k >= NZD - nz);
+          const int iz = in ? (k <= nz ? k : k - NZD) : 0, jc = ix0 + j < D.NXm ? j : 0;
+          const Cx<R> v = cx_from<R>(src[(size_t)jc * D.NZm + iz + nz]);
+          return in ? v : Cx<R>{(R)0, (R)0};
         },
         [&](int e) { const int j = e / NZD; return j * LS + (e - j * NZD); });
     __syncthreads();
@@ -212,20 +210,21 @@ __global__ void __launch_bounds__((ZCfg<R, NZD, false>::NT))
     }
   } else {
     const int ix0 = tile * HG;
-    // every load of the block's lines first, then the recovery
+    // every load of the block's lines first, then the recovery; the loads
+    // are unconditional (a lane with no mode reads the column (ix, 0)), as a
+    // load under a branch is waited for where the branch ends
     Cx<double> xv[HG][PER], ev[HG][PER];
 #pragma unroll
     for (int j = 0; j < HG; j++) {
-      const size_t c0 = rb + (size_t)(ix0 + j) * D.NZm + nz;  // the column (ix, 0)
+      const size_t c0 = rb + (size_t)min(ix0 + j, D.NXm - 1) * D.NZm + nz;  // the column (ix, 0)
 #pragma unroll
       for (int q = 0; q < PER; q++) {
         const int k = threadIdx.x + q * NT;
-        xv[j][q] = ev[j][q] = Cx<double>{0.0, 0.0};
-        if (ix0 + j < D.NXm && k < NZD && (k <= nz || k >= NZD - nz)) {
-          const int iz = k <= nz ? k : k - NZD;
-          xv[j][q] = ld(V3 + c0 + iz);
-          ev[j][q] = ld(V1 + c0 + iz);
-        }
+        const bool in = ix0 + j < D.NXm && k < NZD && (k <= nz || k >= NZD - nz);
+        const int iz = in ? (k <= nz ? k : k - NZD) : 0;
+        const Cx<double> x = ld(V3 + c0 + iz), e = ld(V1 + c0 + iz);
+        xv[j][q] = in ? x : Cx<double>{0.0, 0.0};
+        ev[j][q] = in ? e : Cx<double>{0.0, 0.0};
       }
     }
     // the twiddles, while the lines' loads are in flight

@@ -1170,11 +1170,14 @@ __device__ __forceinline__ void shift_column(const Dims &D, const D0Const &fc, Q
   const int m = D.ny, mi = m - 2;
   constexpr int CH = EXS_CHUNK;
   C x1{0, 0}, x2{0, 0}, sx0{0, 0}, sx1{0, 0};
-  rows_ascending<PF_FWD, C>(
-      mi, [&](int i) { return ld(q + (size_t)(i + 2) * nc + col); },
-      [&](int i, const C &v) {
+  // rows travel through the prefetch as stored (float2 in the mixed form) and
+  // are widened where used: a conversion at the load would wait for it there
+  auto wide = [](const QT &r) { return C{(double)r.x, (double)r.y}; };
+  rows_ascending<PF_FWD, QT>(
+      mi, [&](int i) { return q[(size_t)(i + 2) * nc + col]; },
+      [&](int i, const QT &raw) {
         if (i % CH == 0) CK[(size_t)(i / CH) * nc + col] = make_double4(x1.x, x1.y, x2.x, x2.y);
-        C bx = v;
+        C bx = wide(raw);
         if (i >= 2) bx = bx - F0(i, 0) * x2;
         if (i >= 1) bx = bx - F0(i, 1) * x1;
         bx = bx * F0(i, 2);
@@ -1217,17 +1220,19 @@ __device__ __forceinline__ void shift_column(const Dims &D, const D0Const &fc, Q
       };
   for (int c = (mi - 1) / CH; c >= 0; c--) {
     const int c0 = c * CH, n = min(CH, mi - c0);
-    C xr[CH];
+    // the chunk's rows, loaded without a branch (past the last row: the last
+    // row again), as a load under a branch is waited for where the branch ends
+    QT raw[CH];
 #pragma unroll
-    for (int j = 0; j < CH; j++)
-      if (j < n) xr[j] = ld(q + (size_t)(c0 + j + 2) * nc + col);
+    for (int j = 0; j < CH; j++) raw[j] = q[(size_t)(min(c0 + j, mi - 1) + 2) * nc + col];
     const double4 ck = CK[(size_t)c * nc + col];
     C y1{ck.x, ck.y}, y2{ck.z, ck.w};
+    C xr[CH];
 #pragma unroll
     for (int j = 0; j < CH; j++) {
       if (j < n) {
         const int i = c0 + j;
-        C bx = xr[j];
+        C bx = wide(raw[j]);
         if (i >= 2) bx = bx - F0(i, 0) * y2;
         if (i >= 1) bx = bx - F0(i, 1) * y1;
         bx = bx * F0(i, 2);
