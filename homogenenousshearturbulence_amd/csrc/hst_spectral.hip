@@ -905,17 +905,50 @@ int launch_d0solve(const Dims &D, int rhs_mode, int out_mode, const double2 *src
 // same for every column; sig = [i][2]).  Per point v is read once and dv/dy
 // written once: 32 B against the 64 B of k_d0solve's stored sweep.
 constexpr int DYW_B = 8, DYW_W = 32, DYW_N = DYW_B + DYW_W;
+// CL: the per-row coefficients (the D1 taps -2..2 and the F0 columns 0, 1,
+// 2, 7, 8, 3, 4: DYC doubles a row) read from a copy in LDS.  Read as scalar
+// loads, each row's were waited for one after another (about 50 waits a
+// block of rows, the wavefronts' only other work their own recurrences):
+// 0.81 against 0.64 ms at nx = nz = 256, ny = 513.  CL = 0 where the copy
+// does not fit 64 KB (ny > 684).
+constexpr int DYC = 12;
 
+template <int CL>
 __global__ void __launch_bounds__(256, 2)
     k_dy_window(Dims D, D0Const fc, const double *__restrict__ f0, const double *__restrict__ der,
                 const double2 *__restrict__ src, double2 *__restrict__ dst, const double *__restrict__ sig, double sx,
                 double sz) {
+  const int m = D.ny, mi = m - 2;
+  extern __shared__ double dyc[];
+  if constexpr (CL) {
+    const int n = mi * DYC;
+    for (int b0 = 0; b0 < n; b0 += 16 * 256) {
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int k = min(b0 + u * 256 + (int)threadIdx.x, n - 1), i = k / DYC, j = k - i * DYC;
+        const int fj = j == 5 ? 0 : (j == 6 ? 1 : (j == 7 ? 2 : (j == 8 ? 7 : (j == 9 ? 8 : (j == 10 ? 3 : 4)))));
+        v[u] = j < 5 ? DER(der, i, 1, j - 2) : F0(i, fj);
+      }
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int k = b0 + u * 256 + threadIdx.x;
+        if (k < n) dyc[k] = v[u];
+      }
+    }
+    __syncthreads();
+  }
+  // coefficient j of row i: 0..4 the D1 taps, 5..11 F0 columns 0, 1, 2, 7, 8, 3, 4
+  auto coef = [&](int i, int j) -> double {
+    if constexpr (CL) return dyc[i * DYC + j];
+    constexpr int fj[7] = {0, 1, 2, 7, 8, 3, 4};
+    return j < 5 ? DER(der, i, 1, j - 2) : F0(i, fj[j - 5]);
+  };
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= D.ncol) return;
   int ix, iz;
   col_ix_iz(D, col, ix, iz);
   const size_t nc = D.ncol;
-  const int m = D.ny, mi = m - 2;
   C w[5];  // the D1 stencil's window src(i - 2 .. i + 2)
 #pragma unroll
   for (int k = 0; k < 4; k++) w[k + 1] = ld(src + (size_t)k * nc + col);
@@ -925,12 +958,12 @@ __global__ void __launch_bounds__(256, 2)
     w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4]; w[4] = v;
     C bx{0, 0};
 #pragma unroll
-    for (int j = -2; j <= 2; j++) bx = bx + DER(der, i, 1, j) * w[j + 2];
-    if (i >= 2) bx = bx - F0(i, 0) * x2;
-    if (i >= 1) bx = bx - F0(i, 1) * x1;
-    bx = bx * F0(i, 2);
-    sx0 = sx0 + F0(i, 7) * bx;
-    sx1 = sx1 + F0(i, 8) * bx;
+    for (int j = -2; j <= 2; j++) bx = bx + coef(i, j + 2) * w[j + 2];
+    if (i >= 2) bx = bx - coef(i, 5) * x2;
+    if (i >= 1) bx = bx - coef(i, 6) * x1;
+    bx = bx * coef(i, 7);
+    sx0 = sx0 + coef(i, 8) * bx;
+    sx1 = sx1 + coef(i, 9) * bx;
     x2 = x1;
     x1 = bx;
     return bx;
@@ -949,7 +982,7 @@ __global__ void __launch_bounds__(256, 2)
 #pragma unroll
     for (int k = DYW_N - 1; k >= 0; k--) {
       const int i = r + k;
-      const C h = ring[k] - F0(i, 3) * h1 - F0(i, 4) * h2;
+      const C h = ring[k] - coef(i, 10) * h1 - coef(i, 11) * h2;
       if (k < DYW_B) st(dst + (size_t)(i + 2) * nc + col, h);
       h2 = h1;
       h1 = h;
@@ -1010,18 +1043,26 @@ __global__ void __launch_bounds__(256, 2)
       xk1 = xk;
     }
   }
-  // the border columns' share of the first rows
-  for (int i = 0; i < DYW_W; i++) {
-    const size_t o = (size_t)(i + 2) * nc + col;
-    st(dst + o, ld(dst + o) + sig[2 * i] * yb1 + sig[2 * i + 1] * yb2);
-  }
+  // the border columns' share of the first rows, every load issued before
+  // the first store (a store between two loads of dst made each load wait
+  // for the one before: DYW_W round trips one after another)
+  C fx[DYW_W];
+#pragma unroll
+  for (int i = 0; i < DYW_W; i++) fx[i] = ld(dst + (size_t)(i + 2) * nc + col);
+#pragma unroll
+  for (int i = 0; i < DYW_W; i++)
+    st(dst + (size_t)(i + 2) * nc + col, fx[i] + sig[2 * i] * yb1 + sig[2 * i + 1] * yb2);
 }
 
 int launch_dy_window(const Dims &D, const double2 *src, double2 *dst, const double *f0rows, const D0Const &fc,
                      const double *der, const double *sig, double sx, double sz, hipStream_t s) {
   if (D.ny - 2 < 2 * DYW_N + DYW_W) return hipErrorInvalidValue;
   dim3 grid((D.ncol + 255) / 256);
-  hipLaunchKernelGGL(k_dy_window, grid, dim3(256), 0, s, D, fc, f0rows, der, src, dst, sig, sx, sz);
+  const size_t lds = (size_t)(D.ny - 2) * DYC * sizeof(double);
+  if (lds <= 65536)
+    hipLaunchKernelGGL(k_dy_window<1>, grid, dim3(256), lds, s, D, fc, f0rows, der, src, dst, sig, sx, sz);
+  else
+    hipLaunchKernelGGL(k_dy_window<0>, grid, dim3(256), 0, s, D, fc, f0rows, der, src, dst, sig, sx, sz);
   return hipGetLastError();
 }
 
