@@ -1350,6 +1350,9 @@ int launch_exact_shift(const Dims &D, int nq, void *const *q, const int *f32, do
 // One system alone (half the launch) takes 0.48 of the time of both: the
 // kernel is bound by HBM bandwidth, not by the second, partly filled round.
 constexpr int SP_WAVES = 3, SP_PF = 2;
+#ifndef SP_LDS
+#define SP_LDS 1
+#endif
 template <int K>
 __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, int ix, int iz, int g,
                                                  const D0Const &fcr,
@@ -1485,10 +1488,26 @@ struct ShiftPentaArgs {
 };
 
 // blockIdx.y: system 0 (D2V on rhs2) or 1 (ETA on rhs1)
+// CL: the reversed D0 factor rows read from a copy in LDS (9 doubles a row)
+template <int CL>
 __global__ void __launch_bounds__(256, SP_WAVES)
-    k_shift_penta(Dims D, ShiftPentaArgs a, D0Const fcr, const double *__restrict__ f0r,
+    k_shift_penta(Dims D, ShiftPentaArgs a, D0Const fcr, const double *__restrict__ f0g,
                   const double *__restrict__ der, const double2 *__restrict__ PX, const double2 *__restrict__ PZ,
                   double sx0s, double sz0s, double sx1s, double sz1s, double lam) {
+  extern __shared__ double f0s[];
+  if constexpr (CL) {
+    const int n = (D.ny - 2) * 9;
+    for (int b0 = 0; b0 < n; b0 += 16 * 256) {
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) v[u] = f0g[min(b0 + u * 256 + (int)threadIdx.x, n - 1)];
+#pragma unroll
+      for (int u = 0; u < 16; u++)
+        if (b0 + u * 256 + (int)threadIdx.x < n) f0s[b0 + u * 256 + threadIdx.x] = v[u];
+    }
+    __syncthreads();
+  }
+  const double *const f0r = CL ? f0s : f0g;
   // one column per lane in column order: a wavefront's row is one run of 64
   // columns (the +iz, -iz pairs of pair_lane share the table rows but read two
   // runs a row, measured 1.5 % slower here)
@@ -1521,7 +1540,13 @@ int launch_shift_penta(const Dims &D, const double2 *q2, const double2 *q1, doub
     a.tab[k] = (const double4 *)tabs[k];
   }
   dim3 grid((D.ncol + 255) / 256, 2);
-  hipLaunchKernelGGL(k_shift_penta, grid, dim3(256), 0, s, D, a, fcr, f0r, der, PX, PZ, sx0, sz0, sx1, sz1, lambda);
+  const size_t lds = (size_t)(D.ny - 2) * 9 * sizeof(double);
+  if (SP_LDS && lds <= 49152)
+    hipLaunchKernelGGL(k_shift_penta<1>, grid, dim3(256), lds, s, D, a, fcr, f0r, der, PX, PZ, sx0, sz0, sx1, sz1,
+                       lambda);
+  else
+    hipLaunchKernelGGL(k_shift_penta<0>, grid, dim3(256), 0, s, D, a, fcr, f0r, der, PX, PZ, sx0, sz0, sx1, sz1,
+                       lambda);
   return hipGetLastError();
 }
 
