@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(256)
     if (masked) {  // damping_mask (src/hst_damping.f90:119-133), before the shift
       r1 = C{0, 0}; r2 = C{0, 0}; o1 = C{0, 0}; o2 = C{0, 0};
     }
-    st(R1 + c0, r1); st(R2 + c0, r2); st(O1 + c0, o1); st(O2 + c0, o2);
+    stn(R1 + c0, r1); stn(R2 + c0, r2); stn(O1 + c0, o1); stn(O2 + c0, o2);
   }
 }
 
