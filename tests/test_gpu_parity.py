@@ -580,3 +580,22 @@ def test_dvdy_one_sweep(O, exact, ystretch):
         g.close()
     field = d.ny * (d.nx + 1) * (2 * d.nz + 1) * 16.0
     assert one == pytest.approx(2 * field) and two == pytest.approx(4 * field)
+
+
+def test_long_lines_coefficients_from_global(O):
+    """ny = 701: the per-row coefficient copies in LDS of the fused shift and
+    solve (9 doubles a row, up to 48 KB) and of the one-sweep dv/dy (12 a row,
+    up to 64 KB) no longer fit, and both kernels read the coefficients from
+    global memory instead; two FP64 exact-shift steps of a thin deck against
+    the oracle."""
+    d = dataclasses.replace(read_input(deck_path("small")), nx=6, nz=6, ny=701, kpeak=3.0, alfa0=1.0, beta0=1.0,
+                            ly=2 * np.pi, re=1000.0, amplitude=0.002, deltat=0.001, exact_shift=True,
+                            ystretch=0.0)
+    assert (d.ny - 2) * 9 * 8 > 48 * 1024 and (d.ny - 2) * 12 * 8 > 64 * 1024
+    o = O.run_deck(d, nsteps=2)
+    g = hst.run_deck(d, hst.FP64, nsteps=2)
+    try:
+        assert g.time == o.time
+        assert rel(g.V, o.V) <= 1e-11
+    finally:
+        g.close()
