@@ -74,9 +74,10 @@ void free_fft_tables(void *p) {
 
 // Lines per block of the z kernels, measured at nzd = 768: FP32 four (more
 // blocks per CU than eight, the XCD order keeps the tiles that share a line in
-// one L2); FP64 two in both directions, with the twiddles read through the
-// cache instead of copied to LDS (an FP64 table is 12 KB per block, which
-// would cut the blocks per CU; FP32 keeps its LDS copy, measured faster).
+// one L2); FP64 two in both directions (four: 1.5x slower).  Both copy the
+// twiddles to LDS (FP64 12 KB a block): read through the cache inside the
+// stages, each stage waited for its loads (FP64 k_zift 1.75 -> 1.58-1.70 ms,
+// k_zfft 2.43 -> 2.29-2.33 ms with the copy).
 #ifndef HST_ZG64_FWD
 #define HST_ZG64_FWD 2
 #endif
@@ -84,7 +85,7 @@ void free_fft_tables(void *p) {
 #define HST_ZG64_INV 2
 #endif
 #ifndef HST_ZTW_LDS64
-#define HST_ZTW_LDS64 0
+#define HST_ZTW_LDS64 1
 #endif
 #ifndef HST_ZTW_LDS32
 #define HST_ZTW_LDS32 1
