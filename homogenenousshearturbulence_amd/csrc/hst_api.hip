@@ -101,6 +101,9 @@ struct State {
   // fast enough on this grid; HST_NO_DYWIN=1: the stored sweep of k_d0solve
   bool dyw = false;
   double *d_sig = nullptr;
+  // the exact shift of oldrhs in one sweep (k_shift_window) under the same
+  // condition; HST_NO_SHWIN=1: the chunked two-sweep k_exact_shift
+  bool shw = false;
   // a time step leaves eta in V1 and dv/dy in V3 (every row) instead of u
   // and w: the next step's transforms and right-hand sides read that form,
   // and every other call restores u, w first (restore_uw)
@@ -491,8 +494,12 @@ static int exact_shift_split(double dt_sub, double time, bool fused = false) {
   const bool serial = S.prof || S.serial;
   hipStream_t st = serial ? S.stream : S.side;
   if (!serial) HIPCHK(hipStreamWaitEvent(S.side, S.ev_fork, 0));
-  // per array q in twice and out once, FP32 in the mixed form
-  rc = run_on(st, "exact_shift_old", 2.0 * (3.0 * os) / 16.0 * bytes_field(), [&] {
+  // per array q in twice and out once, FP32 in the mixed form; in one sweep
+  // (k_shift_window) in once and out once
+  rc = run_on(st, "exact_shift_old", 2.0 * ((S.shw ? 2.0 : 3.0) * os) / 16.0 * bytes_field(), [&] {
+    if (S.shw)
+      return launch_shift_window(S.D, 2, qo, fo, S.XRo, S.d_f0, S.fc, S.d_der, S.d_sig, S.PX, S.PZ, sx0, sz0, sx1,
+                                 sz1, st);
     return launch_exact_shift(S.D, 2, qo, fo, S.XRo, S.d_f0, S.fc, S.d_der, S.PX, S.PZ, sx0, sz0, sx1, sz1, st);
   });
   if (rc) return rc;
@@ -887,11 +894,15 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
   HIPCHK(hipMemcpy(S.d_f0r, S.f0r.rows.data(), S.f0r.rows.size() * sizeof(double), hipMemcpyHostToDevice));
   {
     std::vector<double> sig;
-    const char *e = std::getenv("HST_NO_DYWIN");
-    if (!(e && std::atoi(e) != 0) && dy_window_setup(sig)) {
+    auto off = [](const char *name) {
+      const char *e = std::getenv(name);
+      return e && std::atoi(e) != 0;
+    };
+    if (dy_window_setup(sig)) {
       HIPCHK(hipMalloc(&S.d_sig, sig.size() * sizeof(double)));
       HIPCHK(hipMemcpy(S.d_sig, sig.data(), sig.size() * sizeof(double), hipMemcpyHostToDevice));
-      S.dyw = true;
+      S.dyw = !off("HST_NO_DYWIN");
+      S.shw = !off("HST_NO_SHWIN");
     }
   }
   S.part_cap = std::max<size_t>((size_t)D.NYg * D.nzd, (size_t)((D.ncol + 255) / 256) * NSTAT + 64);

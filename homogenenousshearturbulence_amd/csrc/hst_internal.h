@@ -146,6 +146,13 @@ int launch_line_cols(const Dims &D, int kind, double lambda, int col0, int ncols
 int launch_exact_shift(const Dims &D, int nq, void *const *q, const int *f32, double2 *const *XR,
                        const double *f0rows, const D0Const &fc, const double *der, const double2 *PX,
                        const double2 *PZ, double sx0, double sz0, double sx1, double sz1, hipStream_t s);
+// The same shift in one sweep per array (the D0 solve windowed as in
+// launch_dy_window, sig its border terms): q read and written once, XR holding
+// the first dy_window_rows() + 4 rows in FP64; needs ny - 2 >= 104.
+int launch_shift_window(const Dims &D, int nq, void *const *q, const int *f32, double2 *const *XR,
+                        const double *f0rows, const D0Const &fc, const double *der, const double *sig,
+                        const double2 *PX, const double2 *PZ, double sx0, double sz0, double sx1, double sz1,
+                        hipStream_t s);
 // The exact shift of rhs2 and rhs1 and their D2V and ETA line solves at lambda
 // in one launch (v2, eta out; the shifted right-hand sides are not stored):
 // f0r, fcr the D0 factorisation of the reversed line (d0_factor of the

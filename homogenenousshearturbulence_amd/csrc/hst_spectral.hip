@@ -1329,6 +1329,297 @@ int launch_exact_shift(const Dims &D, int nq, void *const *q, const int *f32, do
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ exact shift in one sweep
+
+// shift_unweighted of one array per lane in one sweep, the D0 solve windowed
+// as in k_dy_window: the forward elimination keeps the last SHW_N eliminated
+// rows in registers, and every SHW_B rows the back substitution runs down that
+// window from a zero start and yields its lowest SHW_B rows of x.  From row
+// SHW_W on these are x to rounding and go on at once: x' = x times the node
+// phase, and row j of q = D0 x' is written as soon as x'(j + 2) is known.  The
+// first SHW_S rows of x wait in the FP64 workspace xs (the first SHW_W lack
+// the border terms yb1 s1(i) + yb2 s2(i), known at the end); the top rows take
+// the exact back substitution after the border solve; then the rows of q that
+// read the ghosts (x'(-2), x'(-1) from the top rows, x'(m), x'(m + 1) from the
+// bottom ones) are written.  q is read and written once (the stash: 2 SHW_S
+// rows of 16 B), against twice read and once written with the checkpoints of
+// k_exact_shift: half the bytes for the mixed form's FP32 oldrhs.
+// CL: the D0 taps and F0 columns 0, 1, 2, 7, 8, 3, 4 of each row (SHC doubles)
+// read from a copy in LDS (ny <= 683).
+constexpr int SHW_B = 4, SHW_W = DYW_W, SHW_N = SHW_B + SHW_W, SHW_S = SHW_W + 4, SHC = 12;
+
+template <typename QT, int CL>
+__device__ __forceinline__ void shift_window_column(const Dims &D, const D0Const &fc, QT *__restrict__ q,
+                                                    double2 *__restrict__ xs, int col, const double *__restrict__ f0,
+                                                    const double *cfs, const double *__restrict__ der,
+                                                    const double *__restrict__ sig, const double2 *__restrict__ PX,
+                                                    const double2 *__restrict__ PZ, double sx0s, double sz0s,
+                                                    double sx1s, double sz1s) {
+  int ix, iz;
+  col_ix_iz(D, col, ix, iz);
+  const size_t nc = D.ncol;
+  const int m = D.ny, mi = m - 2;
+  // D0 tap t of row j (j < m); F0 column of row i (i < mi) by its place in the copy
+  auto tap = [&](int j, int t) -> double {
+    if constexpr (CL) return cfs[j * SHC + t + 2];
+    return DER(der, j, 0, t);
+  };
+  auto f0c = [&](int i, int c) -> double {
+    if constexpr (CL) return cfs[i * SHC + c];
+    constexpr int fj[12] = {0, 0, 0, 0, 0, 0, 1, 2, 7, 8, 3, 4};
+    return F0(i, fj[c]);
+  };
+  auto wide = [](const QT &r) { return C{(double)r.x, (double)r.y}; };
+  auto phase = [&](const C &px, const C &pz) { return px * pz; };
+  C x1{0, 0}, x2{0, 0}, sx0{0, 0}, sx1{0, 0};
+  auto fwd = [&](int i, const C &v) {
+    C bx = v;
+    if (i >= 2) bx = bx - f0c(i, 5) * x2;
+    if (i >= 1) bx = bx - f0c(i, 6) * x1;
+    bx = bx * f0c(i, 7);
+    sx0 = sx0 + f0c(i, 8) * bx;
+    sx1 = sx1 + f0c(i, 9) * bx;
+    x2 = x1;
+    x1 = bx;
+    return bx;
+  };
+  C ring[SHW_N];  // ring[k]: the eliminated row r + k
+  {
+    QT v[SHW_N];
+#pragma unroll
+    for (int k = 0; k < SHW_N; k++) v[k] = q[(size_t)(k + 2) * nc + col];
+#pragma unroll
+    for (int k = 0; k < SHW_N; k++) ring[k] = fwd(k, wide(v[k]));
+  }
+  // R0..R3 = x'(i - 4 .. i - 1) when x'(i) arrives; row i - 2 of q is then complete
+  C R0{0, 0}, R1{0, 0}, R2{0, 0}, R3{0, 0};
+  auto push = [&](int i, const C &xp) {
+    if (i >= SHW_W + 4) {
+      const int j = i - 2;
+      C acc{0, 0};
+      acc = acc + tap(j, -2) * R0;
+      acc = acc + tap(j, -1) * R1;
+      acc = acc + tap(j, 0) * R2;
+      acc = acc + tap(j, 1) * R3;
+      acc = acc + tap(j, 2) * xp;
+      st(q + (size_t)(j + 2) * nc + col, acc);
+    }
+    R0 = R1; R1 = R2; R2 = R3; R3 = xp;
+  };
+  // the window's lowest SHW_B rows, substituted from its top with a zero start
+  auto emit = [&](int r, C (&hb)[SHW_B]) {
+    C h1{0, 0}, h2{0, 0};
+#pragma unroll
+    for (int k = SHW_N - 1; k >= 0; k--) {
+      const int i = r + k;
+      const C h = ring[k] - f0c(i, 10) * h1 - f0c(i, 11) * h2;
+      if (k < SHW_B) hb[k] = h;
+      h2 = h1;
+      h1 = h;
+    }
+  };
+  auto on_rows = [&](int r, const C (&hb)[SHW_B], const C (&px)[SHW_B], const C (&pz)[SHW_B]) {
+#pragma unroll
+    for (int k = 0; k < SHW_B; k++) {
+      const int i = r + k;
+      if (i < SHW_S) st(xs + (size_t)i * nc + col, hb[k]);
+      if (i >= SHW_W) push(i, hb[k] * phase(px[k], pz[k]));
+    }
+  };
+  int r = 0;
+  for (; r + SHW_N + SHW_B <= mi; r += SHW_B) {
+    QT v[SHW_B];
+    C px[SHW_B], pz[SHW_B], hb[SHW_B];
+#pragma unroll
+    for (int k = 0; k < SHW_B; k++) v[k] = q[(size_t)(r + SHW_N + k + 2) * nc + col];
+#pragma unroll
+    for (int k = 0; k < SHW_B; k++) {
+      px[k] = ld(PX + (size_t)(r + k) * D.NXm + ix);
+      pz[k] = ld(PZ + (size_t)(r + k) * D.NZm + iz + D.nz);
+    }
+    emit(r, hb);
+    on_rows(r, hb, px, pz);
+#pragma unroll
+    for (int k = 0; k < SHW_W; k++) ring[k] = ring[k + SHW_B];
+#pragma unroll
+    for (int k = 0; k < SHW_B; k++) ring[SHW_W + k] = fwd(r + SHW_N + k, wide(v[k]));
+  }
+  // the last, partial block: rem < SHW_B rows are left to eliminate (loads
+  // clamped to the last row rather than under a branch)
+  const int rem = mi - (r + SHW_N);
+  {
+    QT v[SHW_B];
+    C px[SHW_B], pz[SHW_B], hb[SHW_B];
+#pragma unroll
+    for (int k = 0; k < SHW_B; k++) v[k] = q[(size_t)(min(r + SHW_N + k, mi - 1) + 2) * nc + col];
+#pragma unroll
+    for (int k = 0; k < SHW_B; k++) {
+      px[k] = ld(PX + (size_t)(r + k) * D.NXm + ix);
+      pz[k] = ld(PZ + (size_t)(r + k) * D.NZm + iz + D.nz);
+    }
+    emit(r, hb);
+    on_rows(r, hb, px, pz);
+#pragma unroll
+    for (int k = 0; k < SHW_W; k++) ring[k] = ring[k + SHW_B];
+#pragma unroll
+    for (int k = 0; k < SHW_B; k++)
+      if (k < rem) ring[SHW_W + k] = fwd(r + SHW_N + k, wide(v[k]));
+    r += SHW_B;
+  }
+  // the border system; q's border rows are not written yet (every row written
+  // so far lies below r - 2)
+  const C r25 = wide(q[(size_t)(m - 2 + 2) * nc + col]), r27 = wide(q[(size_t)(m - 1 + 2) * nc + col]);
+  const C ph0 = cexpi(-(D.alfa0 * ix * sx0s + D.beta0 * iz * sz0s));
+  C b1, b2;
+  d0_border(fc, ph0, x1, x2, sx0, sx1, r25, r27, b1, b2);
+  const C cph = conj(ph0);
+  const C yb1 = cph * b1, yb2 = cph * b2;
+  // the rows r .. mi - 1 still in the window: the exact back substitution
+  const int top = mi - r;  // SHW_W + rem of them
+  {
+    C xk1 = b1, xk2 = b2;
+#pragma unroll
+    for (int k = SHW_N - 1; k >= 0; k--) {
+      if (k < top) {
+        const int i = r + k;
+        const C xk = ring[k] - F0(i, 5) * yb1 - F0(i, 6) * yb2 - F0(i, 3) * xk1 - F0(i, 4) * xk2;
+        ring[k] = xk;
+        xk2 = xk1;
+        xk1 = xk;
+      }
+    }
+  }
+  // their x', then the border rows' (x(m - 2) = b1, x(m - 1) = b2), in order
+#pragma unroll
+  for (int k0 = 0; k0 < SHW_N; k0 += SHW_B) {
+    C px[SHW_B], pz[SHW_B];
+#pragma unroll
+    for (int k = 0; k < SHW_B; k++) {
+      const int i = min(r + k0 + k, mi - 1);
+      px[k] = ld(PX + (size_t)i * D.NXm + ix);
+      pz[k] = ld(PZ + (size_t)i * D.NZm + iz + D.nz);
+    }
+#pragma unroll
+    for (int k = 0; k < SHW_B; k++)
+      if (k0 + k < top) push(r + k0 + k, ring[k0 + k] * phase(px[k], pz[k]));
+  }
+  push(m - 2, b1 * phase(ld(PX + (size_t)(m - 2) * D.NXm + ix), ld(PZ + (size_t)(m - 2) * D.NZm + iz + D.nz)));
+  push(m - 1, b2 * phase(ld(PX + (size_t)(m - 1) * D.NXm + ix), ld(PZ + (size_t)(m - 1) * D.NZm + iz + D.nz)));
+  // R0..R3 = x'(m - 4 .. m - 1)
+  const C ph1 = cexpi(-(D.alfa0 * ix * sx1s + D.beta0 * iz * sz1s));
+  const C T0 = R0, T1 = R1, T2 = R2, T3 = R3;
+  // the first SHW_S rows: x from the stash plus, for the first SHW_W, the border
+  // terms; rows 0 .. SHW_W + 1 of q as their x' come, with x'(-2) = x'(m - 2)
+  // conj(ph1) and x'(-1) = x'(m - 1) conj(ph1) ahead of them in the window
+  R2 = T2 * conj(ph1);
+  R3 = T3 * conj(ph1);
+  C f0x{0, 0}, f1x{0, 0};
+#pragma unroll
+  for (int i0 = 0; i0 < SHW_S; i0 += SHW_B) {
+    C z[SHW_B], px[SHW_B], pz[SHW_B];
+#pragma unroll
+    for (int k = 0; k < SHW_B; k++) {
+      const int i = i0 + k;
+      z[k] = ld(xs + (size_t)i * nc + col);
+      px[k] = ld(PX + (size_t)i * D.NXm + ix);
+      pz[k] = ld(PZ + (size_t)i * D.NZm + iz + D.nz);
+    }
+#pragma unroll
+    for (int k = 0; k < SHW_B; k++) {
+      const int i = i0 + k;
+      C x = z[k];
+      if (i < SHW_W) x = x + sig[2 * i] * yb1 + sig[2 * i + 1] * yb2;
+      const C xp = x * phase(px[k], pz[k]);
+      if (i == 0) f0x = xp;
+      if (i == 1) f1x = xp;
+      if (i >= 2) {
+        const int j = i - 2;
+        C acc{0, 0};
+        acc = acc + tap(j, -2) * R0;
+        acc = acc + tap(j, -1) * R1;
+        acc = acc + tap(j, 0) * R2;
+        acc = acc + tap(j, 1) * R3;
+        acc = acc + tap(j, 2) * xp;
+        st(q + (size_t)(j + 2) * nc + col, acc);
+      }
+      R0 = R1; R1 = R2; R2 = R3; R3 = xp;
+    }
+  }
+  // rows m - 2, m - 1, with x'(m) = x'(0) ph1, x'(m + 1) = x'(1) ph1
+  const C gp0 = f0x * ph1, gp1 = f1x * ph1;
+  {
+    C acc{0, 0};
+    acc = acc + tap(m - 2, -2) * T0;
+    acc = acc + tap(m - 2, -1) * T1;
+    acc = acc + tap(m - 2, 0) * T2;
+    acc = acc + tap(m - 2, 1) * T3;
+    acc = acc + tap(m - 2, 2) * gp0;
+    st(q + (size_t)(m - 2 + 2) * nc + col, acc);
+    acc = C{0, 0};
+    acc = acc + tap(m - 1, -2) * T1;
+    acc = acc + tap(m - 1, -1) * T2;
+    acc = acc + tap(m - 1, 0) * T3;
+    acc = acc + tap(m - 1, 1) * gp0;
+    acc = acc + tap(m - 1, 2) * gp1;
+    st(q + (size_t)(m - 1 + 2) * nc + col, acc);
+  }
+}
+
+template <int CL>
+__global__ void __launch_bounds__(256, 2)
+    k_shift_window(Dims D, D0Const fc, QList ql, const double *__restrict__ f0, const double *__restrict__ der,
+                   const double *__restrict__ sig, const double2 *__restrict__ PX, const double2 *__restrict__ PZ,
+                   double sx0s, double sz0s, double sx1s, double sz1s) {
+  const int m = D.ny, mi = m - 2;
+  extern __shared__ double cfs[];
+  if constexpr (CL) {
+    // [row j < m][SHC]: D0 taps -2..2, then F0 columns 0, 1, 2, 7, 8, 3, 4 (rows < mi)
+    const int n = m * SHC;
+    for (int b0 = 0; b0 < n; b0 += 16 * 256) {
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int k = min(b0 + u * 256 + (int)threadIdx.x, n - 1), j = k / SHC, c = k - j * SHC;
+        const int fj = c == 5 ? 0 : (c == 6 ? 1 : (c == 7 ? 2 : (c == 8 ? 7 : (c == 9 ? 8 : (c == 10 ? 3 : 4)))));
+        v[u] = c < 5 ? DER(der, j, 0, c - 2) : (j < mi ? F0(j, fj) : 0.0);
+      }
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int k = b0 + u * 256 + threadIdx.x;
+        if (k < n) cfs[k] = v[u];
+      }
+    }
+    __syncthreads();
+  }
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= D.ncol) return;
+  const int k = blockIdx.y;
+  if (ql.f32[k])
+    shift_window_column<float2, CL>(D, fc, (float2 *)ql.q[k], ql.xr[k], col, f0, cfs, der, sig, PX, PZ, sx0s, sz0s,
+                                    sx1s, sz1s);
+  else
+    shift_window_column<double2, CL>(D, fc, (double2 *)ql.q[k], ql.xr[k], col, f0, cfs, der, sig, PX, PZ, sx0s,
+                                     sz0s, sx1s, sz1s);
+}
+
+int launch_shift_window(const Dims &D, int nq, void *const *q, const int *f32, double2 *const *XR,
+                        const double *f0rows, const D0Const &fc, const double *der, const double *sig,
+                        const double2 *PX, const double2 *PZ, double sx0, double sz0, double sx1, double sz1,
+                        hipStream_t s) {
+  if (D.ny - 2 < 2 * SHW_N + SHW_W || nq < 1 || nq > 4) return hipErrorInvalidValue;
+  QList ql;
+  for (int k = 0; k < nq; k++) { ql.q[k] = q[k]; ql.xr[k] = XR[k]; ql.f32[k] = f32[k]; }
+  dim3 grid((D.ncol + 255) / 256, nq);
+  const size_t lds = (size_t)D.ny * SHC * sizeof(double);
+  if (lds <= 65536)
+    hipLaunchKernelGGL(k_shift_window<1>, grid, dim3(256), lds, s, D, fc, ql, f0rows, der, sig, PX, PZ, sx0, sz0,
+                       sx1, sz1);
+  else
+    hipLaunchKernelGGL(k_shift_window<0>, grid, dim3(256), 0, s, D, fc, ql, f0rows, der, sig, PX, PZ, sx0, sz0, sx1,
+                       sz1);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ exact shift + line solve
 
 // The exact shift of one right-hand side (shift_unweighted,

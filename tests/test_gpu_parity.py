@@ -582,6 +582,49 @@ def test_dvdy_one_sweep(O, exact, ystretch):
     assert one == pytest.approx(2 * field) and two == pytest.approx(4 * field)
 
 
+@pytest.mark.parametrize("precision,tol", [(hst.FP64, 1e-13), (hst.MIXED, 1e-6)])
+def test_oldrhs_shift_one_sweep(O, precision, tol):
+    """The time step's exact shift of oldrhs in one sweep (k_shift_window,
+    taken with the one-sweep dv/dy): three exact-shift steps of a thin deck
+    with ny = 257 against the oracle (FP64) and against the chunked two-sweep
+    shift (HST_NO_SHWIN=1), on a stretched grid; the profile shows the bytes
+    of the one-sweep kernel (oldrhs read and written once)."""
+    d = dataclasses.replace(read_input(deck_path("small")), nx=6, nz=6, ny=257, kpeak=3.0, alfa0=1.0, beta0=1.0,
+                            ly=2 * np.pi, re=1000.0, amplitude=0.002, deltat=0.001, exact_shift=True,
+                            ystretch=1.5)
+
+    def old_bytes(g):
+        g.profile(True)
+        g.timestep(d.deltat)
+        g.sync()
+        row = [r for r in g.profile_table() if r["kernel"] == "exact_shift_old"]
+        g.profile(False)
+        return row[0]["bytes"] / row[0]["launches"]
+
+    g = hst.run_deck(d, precision, nsteps=3)
+    try:
+        V1 = g.V
+        one = old_bytes(g)
+    finally:
+        g.close()
+    if precision == hst.FP64:
+        o = O.run_deck(d, nsteps=3)
+        assert rel(V1, o.V) <= 1e-11
+    os.environ["HST_NO_SHWIN"] = "1"
+    try:
+        g = hst.run_deck(d, precision, nsteps=3)
+    finally:
+        del os.environ["HST_NO_SHWIN"]
+    try:
+        assert rel(g.V, V1) <= tol
+        two = old_bytes(g)
+    finally:
+        g.close()
+    os_ = 16.0 if precision == hst.FP64 else 8.0
+    field = d.ny * (d.nx + 1) * (2 * d.nz + 1) * os_
+    assert one == pytest.approx(2 * 2 * field) and two == pytest.approx(2 * 3 * field)
+
+
 def test_long_lines_coefficients_from_global(O):
     """ny = 701: the per-row coefficient copies in LDS of the fused shift and
     solve (9 doubles a row, up to 48 KB) and of the one-sweep dv/dy (12 a row,
