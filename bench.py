@@ -110,8 +110,9 @@ def cpu_step(deck, threads):
 
 
 def cpu_baseline(deck, threads, small_deck):
-    """BASELINE.md's CPU plan with the build's FP64 oracle: one whole step of
-    the metric deck at nproc threads (the value), and one whole step of the
+    """The CPU baseline, timed with the build's FP64 oracle after a warm-up
+    transform: one whole step of the metric deck at nproc threads (the value),
+    and one whole step of the
     nx = nz = 32, ny = 65 deck (the CPU-runnable correctness workload) at 1
     thread and at nproc threads: one thread on the metric deck (about 16 times
     the nproc time) or on nx = nz = 128, ny = 257 (over two minutes a step) does
