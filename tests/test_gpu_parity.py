@@ -625,6 +625,30 @@ def test_oldrhs_shift_one_sweep(O, precision, tol):
     assert one == pytest.approx(2 * 2 * field) and two == pytest.approx(2 * 3 * field)
 
 
+@pytest.mark.parametrize("precision", [hst.FP64, hst.MIXED])
+def test_side_stream_matches_serial(precision):
+    """The oldrhs shift runs on a second stream beside the solves, joined
+    before the next right-hand sides; run on the main stream (HST_SERIAL=1)
+    the same kernels must give the same bits: five exact-shift steps of a thin
+    deck with ny = 257 (the one-sweep shift on the second stream)."""
+    d = dataclasses.replace(read_input(deck_path("small")), nx=6, nz=6, ny=257, kpeak=3.0, alfa0=1.0, beta0=1.0,
+                            ly=2 * np.pi, re=1000.0, amplitude=0.002, deltat=0.001, exact_shift=True)
+    g = hst.run_deck(d, precision, nsteps=5)
+    try:
+        V1 = g.V
+    finally:
+        g.close()
+    os.environ["HST_SERIAL"] = "1"
+    try:
+        g = hst.run_deck(d, precision, nsteps=5)
+    finally:
+        del os.environ["HST_SERIAL"]
+    try:
+        assert np.array_equal(g.V, V1)
+    finally:
+        g.close()
+
+
 def test_long_lines_coefficients_from_global(O):
     """ny = 701: the per-row coefficient copies in LDS of the fused shift and
     solve (9 doubles a row, up to 48 KB) and of the one-sweep dv/dy (12 a row,
