@@ -649,20 +649,21 @@ def test_side_stream_matches_serial(precision):
         g.close()
 
 
-def test_long_lines_coefficients_from_global(O):
+@pytest.mark.parametrize("precision,tol", [(hst.FP64, 1e-11), (hst.MIXED, 2e-5)])
+def test_long_lines_coefficients_from_global(O, precision, tol):
     """ny = 701: the per-row coefficient copies in LDS of the fused shift and
-    solve (9 doubles a row, up to 48 KB) and of the one-sweep dv/dy (12 a row,
-    up to 64 KB) no longer fit, and both kernels read the coefficients from
-    global memory instead; two FP64 exact-shift steps of a thin deck against
-    the oracle."""
+    solve (9 doubles a row, up to 48 KB), of the one-sweep dv/dy and of the
+    one-sweep oldrhs shift (12 a row, up to 64 KB) no longer fit, and the
+    kernels read the coefficients from global memory instead; two exact-shift
+    steps of a thin deck against the oracle (FP64 and the mixed form)."""
     d = dataclasses.replace(read_input(deck_path("small")), nx=6, nz=6, ny=701, kpeak=3.0, alfa0=1.0, beta0=1.0,
                             ly=2 * np.pi, re=1000.0, amplitude=0.002, deltat=0.001, exact_shift=True,
                             ystretch=0.0)
     assert (d.ny - 2) * 9 * 8 > 48 * 1024 and (d.ny - 2) * 12 * 8 > 64 * 1024
     o = O.run_deck(d, nsteps=2)
-    g = hst.run_deck(d, hst.FP64, nsteps=2)
+    g = hst.run_deck(d, precision, nsteps=2)
     try:
         assert g.time == o.time
-        assert rel(g.V, o.V) <= 1e-11
+        assert rel(g.V, o.V) <= tol
     finally:
         g.close()
