@@ -59,6 +59,7 @@ __device__ __forceinline__ Cx<PR> ldp(const typename PTraits<PR>::T2 *p) {
 // 1, 2: add the products of group 1 or 2 to rhs, oldrhs as buildrhs(ODE, g);
 // 4: prepare and the five combined products of the x pass's MODE 3
 // (vv - uu, vv - ww, uv, vw, uw), the same sums regrouped.
+constexpr int RHS_ROWS = 64;
 template <typename PR, int MODE, bool SHIFT>
 __global__ void __launch_bounds__(256)
     k_buildrhs(Dims D, RhsArgs a, const double2 *__restrict__ V1, const double2 *__restrict__ V2,
@@ -71,6 +72,28 @@ __global__ void __launch_bounds__(256)
   constexpr bool FIVE = (MODE == 4 || MODE == 5), CMP = (MODE == 5);
   constexpr bool NONLIN = MODE != 0, PREP = (MODE == 0 || MODE == 3 || FIVE);
   constexpr int Q0 = (MODE == 2 ? 3 : 0), Q1 = (MODE == 1 ? 3 : (FIVE ? 5 : 6));
+  const int iy0 = blockIdx.y * rch, iy1 = min(iy0 + rch, D.ny);
+  // FP32 products: the stencil rows of the block's chunk from LDS (read as
+  // scalar loads, each row's waited for in turn; 2.5 % faster).  FP64 keeps
+  // the scalar loads: read from LDS the uniform values take VGPRs (218 -> 254)
+  // and it ran 2 % slower.
+  constexpr bool LROWS = sizeof(PR) == 4;
+  __shared__ double dl[LROWS ? RHS_ROWS * 20 : 1];
+  if constexpr (LROWS) {
+    constexpr int PER = (RHS_ROWS * 20 + 255) / 256;
+    const int n = (iy1 - iy0) * 20;
+    double t[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) t[u] = der[iy0 * 20 + min(u * 256 + (int)threadIdx.x, n - 1)];
+#pragma unroll
+    for (int u = 0; u < PER; u++)
+      if (u * 256 + (int)threadIdx.x < n) dl[u * 256 + threadIdx.x] = t[u];
+    __syncthreads();
+  }
+  auto DERL = [&](int iy, int n, int j) -> double {
+    if constexpr (LROWS) return dl[(iy - iy0) * 20 + n * 5 + j + 2];
+    return DER(der, iy, n, j);
+  };
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= D.ncol) return;
   int ix, iz;
@@ -81,7 +104,6 @@ __global__ void __launch_bounds__(256)
   const double gam = a.s2now * (ix * D.alfa0) - D.S * (iz * D.beta0);
   const double gd = a.gdamp ? a.gsoft * a.gdamp[col] : 0.0;
   const bool masked = a.hard && a.gdamp && a.gdamp[col] > 0.0;
-  const int iy0 = blockIdx.y * rch, iy1 = min(iy0 + rch, D.ny);
   const size_t nc = D.ncol;
   // windows: w[k] holds row iy - 2 + k
   C u[5], v[5], w[5];
@@ -122,7 +144,7 @@ __global__ void __launch_bounds__(256)
     C unkn{0, 0}, impl{0, 0};
 #pragma unroll
     for (int j = 0; j < 5; j++) {
-      const double d0 = DER(der, iy, 0, j - 2), d2 = DER(der, iy, 2, j - 2), d4 = DER(der, iy, 3, j - 2);
+      const double d0 = DERL(iy, 0, j - 2), d2 = DERL(iy, 2, j - 2), d4 = DERL(iy, 3, j - 2);
       const double helm = d2 - kk * d0;
       const double biharm = d4 - 2.0 * kk * d2 + kk * kk * d0;
       unkn = unkn + helm * v[j];
@@ -143,7 +165,7 @@ __global__ void __launch_bounds__(256)
     double force = 0.0;
 #pragma unroll
     for (int j = 0; j < 5; j++) {
-      const double d0 = DER(der, iy, 0, j - 2), d2 = DER(der, iy, 2, j - 2);
+      const double d0 = DERL(iy, 0, j - 2), d2 = DERL(iy, 2, j - 2);
       const double helm = d2 - kk * d0;
       if (m00) {
         const C f = CMP ? u[j] : C{u[j].x, w[j].x};
@@ -175,7 +197,7 @@ __global__ void __launch_bounds__(256)
       for (int q = 0; q < 5; q++) { d0[q] = Cx<PR>{0, 0}; d1[q] = Cx<PR>{0, 0}; d2[q] = Cx<PR>{0, 0}; }
 #pragma unroll
       for (int j = 0; j < 5; j++) {
-        const PR c0 = (PR)DER(der, iy, 0, j - 2), c1 = (PR)DER(der, iy, 1, j - 2), c2 = (PR)DER(der, iy, 2, j - 2);
+        const PR c0 = (PR)DERL(iy, 0, j - 2), c1 = (PR)DERL(iy, 1, j - 2), c2 = (PR)DERL(iy, 2, j - 2);
 #pragma unroll
         for (int q = 0; q < 5; q++) {  // (D2 of P0, P1, P4 is not used and drops out)
           d0[q] = d0[q] + c0 * p[q][j];
@@ -202,9 +224,9 @@ __global__ void __launch_bounds__(256)
       PR e0[5], e1[5], e2[5];
 #pragma unroll
       for (int j = 0; j < 5; j++) {
-        e0[j] = (PR)DER(der, iy, 0, j - 2);
-        e1[j] = (PR)DER(der, iy, 1, j - 2);
-        e2[j] = (PR)DER(der, iy, 2, j - 2);
+        e0[j] = (PR)DERL(iy, 0, j - 2);
+        e1[j] = (PR)DERL(iy, 1, j - 2);
+        e2[j] = (PR)DERL(iy, 2, j - 2);
       }
 #pragma unroll
       for (int q = Q0; q < Q1; q++) {
@@ -253,7 +275,7 @@ __global__ void __launch_bounds__(256)
 int launch_buildrhs(const Dims &D, int mixed, const RhsArgs &a, const double2 *V1, const double2 *V2,
                     const double2 *V3, const void *const P[6], double2 *R1, double2 *R2, void *O1, void *O2,
                     const double *der, const double *y, hipStream_t s) {
-  const int rch = 64;  // rows per lane: the 5-row windows re-read 4 rows per chunk
+  const int rch = RHS_ROWS;  // rows per lane: the 5-row windows re-read 4 rows per chunk
   dim3 grid((D.ncol + 255) / 256, (D.ny + rch - 1) / rch);
   const size_t pstride = (size_t)D.NYg * D.ncol;
   // 0 prepare, 1/2 one group, 3 all, 4 all from the five combinations, 5 as 4
