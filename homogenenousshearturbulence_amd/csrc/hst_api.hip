@@ -826,8 +826,11 @@ int hst_gpu_init(const hst_deck *deck, int precision) {
   HIPCHK(hipMalloc(&S.P, 6 * nf * cs));
   HIPCHK(hipMemset(S.P, 0, 6 * nf * cs));
   held += 6 * nf * cs;
-  // solver workspace: two per-column sweeps (48 B per row each) or four D0 sweeps (16 B)
-  const size_t rows = (size_t)(D.ny - 2) * D.ncol;
+  // solver workspace: two per-column sweeps (48 B per row each) or four D0
+  // sweeps (16 B), laid out with the padded row stride D.ncs (the fused shift
+  // and solve indexes its workspace with it; the other kernels, with ncol,
+  // stay inside each view)
+  const size_t rows = (size_t)(D.ny - 2) * D.ncs;
   S.ws_bytes = std::max(2 * rows * 48, nf * sizeof(double2));
   HIPCHK(hipMalloc(&S.ws, S.ws_bytes));
   held += S.ws_bytes;

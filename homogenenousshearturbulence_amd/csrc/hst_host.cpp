@@ -350,6 +350,7 @@ Dims make_dims(const hst_deck &d, const hst_grid_info &g) {
   Dims D;
   D.nx = d.nx; D.ny = d.ny; D.nz = d.nz;
   D.NXm = g.nxm; D.NZm = g.nzm; D.NYg = g.nyg; D.ncol = g.ncol;
+  D.ncs = (g.ncol + 255) / 256 * 256;
   D.nxd = g.nxd; D.nzd = g.nzd;
   D.NXp = (D.NXm + 15) / 16 * 16;
   D.alfa0 = d.alfa0; D.beta0 = d.beta0; D.ni = g.ni; D.S = d.S;

@@ -1111,6 +1111,7 @@ __global__ void __launch_bounds__(256)
   col_ix_iz(D, col0 + l, ix, iz);
   Dims Ds = D;
   Ds.ncol = ncols;  // the row stride of src, dst and ws
+  Ds.ncs = ncols;
   if constexpr (K == KD0 || K == KDY) {
     const size_t nc = ncols;
     const int m = D.ny;
@@ -1675,7 +1676,7 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, int ix,
                                                  double sx0s, double sz0s, double sx1s, double sz1s, double lam,
                                                  double2 *__restrict__ XRp, const double4 *__restrict__ tab,
                                                  double2 *__restrict__ dst) {
-  const size_t nc = D.ncol;
+  const size_t nc = D.ncol, ws = D.ncs;  // field rows, workspace rows
   const int m = D.ny, mi = m - 2;
   if (ix == 0 && iz == 0 && K == KD2V) {  // the (0,0) D2V line is zero (src/hst_linsolve.f90:199,221-224)
     for (int i = 0; i < m; i++) st(dst + (size_t)(i + 2) * nc + col, C{0.0, 0.0});
@@ -1690,7 +1691,7 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, int ix,
         if (k >= 2) bx = bx - F0(k, 0) * x2;
         if (k >= 1) bx = bx - F0(k, 1) * x1;
         bx = bx * F0(k, 2);
-        st(XRs + (size_t)k * nc + col, bx);
+        st(XRs + (size_t)k * ws + col, bx);
         sx0 = sx0 + F0(k, 7) * bx;
         sx1 = sx1 + F0(k, 8) * bx;
         x2 = x1;
@@ -1726,7 +1727,7 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, int ix,
     if (i >= 2) bx = bx - l2 * px2;
     if (i >= 1) bx = bx - l1 * px1;
     bx = bx * rp;
-    st(XRp + (size_t)i * nc + col, bx);
+    st(XRp + (size_t)i * ws + col, bx);
     psx0 = psx0 + f.ww * bx;
     psx1 = psx1 + f.vv * bx;
     px2 = px1;
@@ -1756,7 +1757,7 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, int ix,
       mi,
       [&](int k) {
         const int i = m - 1 - k;
-        return Row{ld(XRs + (size_t)k * nc + col), ld(PX + (size_t)i * D.NXm + ix),
+        return Row{ld(XRs + (size_t)k * ws + col), ld(PX + (size_t)i * D.NXm + ix),
                    ld(PZ + (size_t)i * D.NZm + az + D.nz)};
       },
       [&](int k, const Row &r) {
@@ -1790,7 +1791,7 @@ __device__ __forceinline__ void shift_penta_line(const Dims &D, int col, int ix,
   const C cph1 = conj(ph1);
   const C zb1 = cph1 * c1, zb2 = cph1 * c2;
   penta_back<K>(
-      D, g, lam, kk, der, tab, zb1, zb2, c1, c2, [&](int i) { return ld(XRp + (size_t)i * nc + col); },
+      D, g, lam, kk, der, tab, zb1, zb2, c1, c2, [&](int i) { return ld(XRp + (size_t)i * ws + col); },
       [&](int i, const C &x) { st(dst + (size_t)(i + 2) * nc + col, x); });
 }
 
