@@ -92,3 +92,16 @@ def test_noalloc_line_solve_needs_init():
     rc = L.hst_gpu_line_solve_noalloc(1, 1.0, 0, 1, None, None, 0.0, 0.0)
     assert rc != 0
     assert "hst_gpu_init" in hst.last_error()
+
+
+def test_bench_runs_on_one_gpu_only():
+    """bench.py has no multi-GPU path: --gpus 2 exits with status 2 and says
+    so, before it touches a GPU, and prints no result line."""
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and "--gpus 2" in r.stderr
+    assert r.stdout.strip() == ""
