@@ -133,6 +133,28 @@ def test_products_power_of_two_lengths(O, nx, precision, tol):
             assert rel(g.product(p + 1), P0[p]) <= tol, p
 
 
+@pytest.mark.parametrize("nx", [64, 256, 512])
+@pytest.mark.parametrize("precision,tol", [(hst.FP64, 1e-11), (hst.MIXED, 2e-5)])
+def test_steps_x_modes_filling_half_the_line(O, nx, precision, tol):
+    """Two time steps vs the oracle where the x modes fill half the x line
+    (nx = nxd / 2, x lengths 2 nxd = 256, 1024, 2048, as in the nx = 128, 256
+    and 512 decks): the register-resident x pass in the time step's form (the
+    five product combinations) at those lengths."""
+    d = dataclasses.replace(read_input(deck_path("small")), nx=nx, nz=4, ny=12, kpeak=3.0)
+    assert 2 * d.nx == d.nxd
+    o = seeded(O, d, 0.17)
+    with GPU(d, precision) as g:
+        g.set_V(o.V)
+        g.time = o.time
+        for _ in range(2):
+            o.timestep(d.deltat)
+            g.timestep(d.deltat)
+        assert g.time == o.time
+        Vg, Vo = g.V, o.V
+    for c in range(3):
+        assert refops.rel_l2(Vg[c][..., 2:-2], Vo[c][..., 2:-2]) <= tol, c
+
+
 @pytest.mark.parametrize("nx", [15, 42, 682])
 @pytest.mark.parametrize("precision,tol", [(hst.FP64, 1e-12), (hst.MIXED, 1e-5)])
 def test_cfl_matches_oracle(O, nx, precision, tol):
